@@ -1,0 +1,113 @@
+/* C ABI of the MI355X (gfx950) IF-RK4 pseudo-spectral step library, libpsx.so.
+ *
+ * The library advances the state of one periodic-box Navier-Stokes flow with the Lawson
+ * integrating-factor RK4 scheme on one GPU: hand-written batched r2c/c2r Stockham FFTs with
+ * the Basdevant nonlinear term, the 2/3 dealias mask, the Leray projection (3D), the
+ * integrating factor exp(L dt) (viscosity, linear drag, constant-power band forcing) and the
+ * RK stage combination fused into the transform kernels.
+ *
+ * Ownership: the caller allocates the state, the accumulator and the work buffer with the
+ * sizes psx_query_layout reports and keeps them alive until psx_destroy; the library borrows
+ * the pointers and owns only its small tables (twiddles, FP64 integrating-factor table,
+ * reduction scratch) and its captured graph.  Every enqueueing call takes the caller's HIP
+ * stream (hipStream_t passed as void*); work is ordered on it.  Calls on one context are not
+ * re-entrant.  Every function returns 0 on success and a non-zero code on failure, with a
+ * message in psx_last_error() (thread-local).
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths in the pyspectral repository).
+ */
+#ifndef PSX_H_
+#define PSX_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct psx_ctx psx_ctx;
+
+/* Physical and numerical parameters of one flow.
+ * Mirrors src/pyspectral/config.py:28-38 (Config) with the derived quantities of
+ * config.py:59-62 (drag_k2) and base.py:114-123 (the band of the linear operator).
+ * form: 0 = basdevant (2D and 3D), 1 = rotational (3D only).  fp64: working precision. */
+typedef struct psx_flow_params {
+  int n;
+  int dim;
+  int form;
+  int fp64;
+  double nu;
+  double alpha;
+  double ka2;  /* drag band upper k^2 (1e30: uniform drag) */
+  double klo2; /* forcing band lower k^2 */
+  double khi2; /* forcing band upper k^2 */
+} psx_flow_params;
+
+/* Buffer sizes in complex elements of the working precision. */
+typedef struct psx_layout {
+  long long state_elems; /* u0 and acc each: n^(d-1) (n/2+1) (x3 in 3D), NumPy r2c layout */
+  long long work_elems;  /* work buffer: nfields * n^(d-1) * ldw */
+  int nfields;           /* work fields: 2 (2D), 5 (3D basdevant), 6 (3D rotational) */
+  int ldw;               /* padded retained half-axis width of a work row */
+  int jm;                /* retained half-axis modes, 3 j < n */
+  int four_step;         /* 1 if the full axes use the two-kernel split */
+} psx_layout;
+
+/* Plan the kernels for p and report the buffer sizes.  Replaces the buffer and plan set-up
+ * of src/pyspectral/backends/cupy_cuda.py:40-82,100-112 (CupyCudaSolver.__init__,
+ * _make_plans).  Returns non-zero for unsupported sizes (n must be even, >= 8, of the
+ * form 2^a or 3*2^a). */
+int psx_query_layout(const psx_flow_params* p, psx_layout* out);
+
+/* Create a context on the current device borrowing u0, acc and work.
+ * Replaces cupy_cuda.py:40-82 (buffers, NVRTC module, cuFFT plans). */
+int psx_create(const psx_flow_params* p, void* u0, void* acc, void* work, psx_ctx** out);
+
+/* Release the context (not the borrowed buffers). */
+int psx_destroy(psx_ctx* c);
+
+/* After the caller wrote a masked spectral state into u0: compute the first stage's
+ * transform inputs.  Replaces cupy_cuda.py:134-145 (set_state/_pre, kernel pre2d/pre3d of
+ * src/pyspectral/kernels/spectral.cu:93-103,164-173). */
+int psx_prepare(psx_ctx* c, void* stream);
+
+/* Set the step size and the band-forcing growth rate for the following steps: the FP64
+ * integrating-factor tables are computed on the device into a parameter block the step
+ * kernels read, so a captured step graph stays valid.  Replaces cupy_cuda.py:151-161
+ * (_update_table) and src/pyspectral/backends/_fused_common.py:10-19 (if_table, if_scalars). */
+int psx_set_step(psx_ctx* c, double dt, double amp, void* stream);
+
+/* One IF-RK4 step (4 stages) with the current parameters.  Replaces cupy_cuda.py:163-195
+ * (_stage, _step without graph). */
+int psx_step(psx_ctx* c, void* stream);
+
+/* Same step replayed from a hipGraph captured on first use (dt and amp live in device
+ * memory, so the graph is captured once).  Replaces cupy_cuda.py:181-195 (graph variant). */
+int psx_step_graph(psx_ctx* c, void* stream);
+
+/* max |u_i| over the physical grid of the current state, per component (2 in 2D, 3 in 3D);
+ * synchronises the stream.  Replaces src/pyspectral/diagnostics.py:99-109
+ * (max_velocity_sum, which sums these). */
+int psx_velocity_max(psx_ctx* c, double* out3, void* stream);
+
+/* FP64 energy, enstrophy, palinstrophy and band energy (klo^2 <= k^2 <= khi^2) of a state
+ * in NumPy layout with this context's size and precision; synchronises the stream.
+ * Replaces src/pyspectral/diagnostics.py:65-88. */
+int psx_reduce(psx_ctx* c, const void* state, double klo2, double khi2, double* out4,
+               void* stream);
+
+/* FP64 shell spectrum E(k), k = 0..nshell-1 (shell = int(|k| + 0.5)); synchronises.
+ * Replaces src/pyspectral/diagnostics.py:90-97. */
+int psx_spectrum(psx_ctx* c, const void* state, double* out, int nshell, void* stream);
+
+/* Message of the last failure on this thread. */
+const char* psx_last_error(void);
+
+/* Device name and architecture of the current device into buf; 0 if a gfx950 device is
+ * present.  Replaces the availability probe of cupy_cuda.py:84-93. */
+int psx_device_info(char* buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSX_H_ */

@@ -1,0 +1,79 @@
+"""ctypes binding of ``libpsx.so`` (declarations: include/psx.h).
+
+The library is built in-tree by ``pyspectral_amd.build``; nothing here falls back to another
+implementation: a missing or unloadable library raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from ctypes import POINTER, c_char_p, c_double, c_int, c_longlong, c_void_p
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libpsx.so"
+
+
+class FlowParams(ctypes.Structure):
+    _fields_ = [("n", c_int), ("dim", c_int), ("form", c_int), ("fp64", c_int),
+                ("nu", c_double), ("alpha", c_double), ("ka2", c_double), ("klo2", c_double),
+                ("khi2", c_double)]
+
+
+class Layout(ctypes.Structure):
+    _fields_ = [("state_elems", c_longlong), ("work_elems", c_longlong), ("nfields", c_int),
+                ("ldw", c_int), ("jm", c_int), ("four_step", c_int)]
+
+
+# name -> (restype, argtypes); the exported C ABI
+SIGNATURES = {
+    "psx_query_layout": (c_int, [POINTER(FlowParams), POINTER(Layout)]),
+    "psx_create": (c_int, [POINTER(FlowParams), c_void_p, c_void_p, c_void_p,
+                           POINTER(c_void_p)]),
+    "psx_destroy": (c_int, [c_void_p]),
+    "psx_prepare": (c_int, [c_void_p, c_void_p]),
+    "psx_set_step": (c_int, [c_void_p, c_double, c_double, c_void_p]),
+    "psx_step": (c_int, [c_void_p, c_void_p]),
+    "psx_step_graph": (c_int, [c_void_p, c_void_p]),
+    "psx_velocity_max": (c_int, [c_void_p, POINTER(c_double), c_void_p]),
+    "psx_reduce": (c_int, [c_void_p, c_void_p, c_double, c_double, POINTER(c_double),
+                           c_void_p]),
+    "psx_spectrum": (c_int, [c_void_p, c_void_p, POINTER(c_double), c_int, c_void_p]),
+    "psx_last_error": (c_char_p, []),
+    "psx_device_info": (c_int, [c_char_p, c_int]),
+}
+
+_LIB: ctypes.CDLL | None = None
+
+
+def load() -> ctypes.CDLL:
+    """Load ``libpsx.so`` once and declare its signatures."""
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m pyspectral_amd.build`")
+        lib = ctypes.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = lib
+    return _LIB
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load().psx_last_error()
+        raise RuntimeError((msg or b"psx error").decode())
+
+
+def stream_ptr() -> c_void_p:
+    """The current torch (HIP) stream as a ``hipStream_t``."""
+    import torch
+
+    return c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def device_info() -> tuple[bool, str]:
+    buf = ctypes.create_string_buffer(256)
+    rc = load().psx_device_info(buf, 256)
+    return rc == 0, buf.value.decode() if rc == 0 else (load().psx_last_error() or b"").decode()

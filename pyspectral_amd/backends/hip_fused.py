@@ -1,0 +1,171 @@
+"""MI355X variants ``hip/fused`` and ``hip/fused-graph``: the IF-RK4 step on libpsx.
+
+Host side of the path the reference implements in src/pyspectral/backends/cupy_cuda.py:32-213.
+Torch owns the device buffers (state, RK accumulator, work fields); the C context borrows
+their pointers and owns its plans, FP64 parameter block and captured graph.  Every call is
+enqueued on torch's current stream.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from typing import Any
+
+import numpy as np
+
+from pyspectral_amd import _lib
+from pyspectral_amd.backends.base import BackendUnavailable, SpectralSolver
+from pyspectral_amd.config import Config
+from pyspectral_amd.grid import retained
+from pyspectral_amd.ops import TorchOps
+
+_FORM_CODE = {"basdevant": 0, "rotational": 1}
+
+
+def flow_params(cfg: Config, form: str) -> _lib.FlowParams:
+    return _lib.FlowParams(n=cfg.n, dim=cfg.dim, form=_FORM_CODE[form], fp64=int(cfg.is_fp64),
+                           nu=cfg.nu, alpha=cfg.alpha, ka2=cfg.drag_k2,
+                           klo2=cfg.force_klo**2, khi2=cfg.force_khi**2)
+
+
+class _Buffers:
+    """Device buffers plus the C context that borrows them."""
+
+    def __init__(self, params: _lib.FlowParams, state_shape: tuple[int, ...], cdt: Any) -> None:
+        import torch
+
+        lib = _lib.load()
+        lay = _lib.Layout()
+        if lib.psx_query_layout(ctypes.byref(params), ctypes.byref(lay)) != 0:
+            raise BackendUnavailable(lib.psx_last_error().decode())
+        self.layout = lay
+        self.u0 = torch.zeros(state_shape, dtype=cdt, device="cuda")
+        self.acc = torch.zeros_like(self.u0)
+        self.work = torch.zeros(lay.work_elems, dtype=cdt, device="cuda")
+        ctx = ctypes.c_void_p()
+        _lib.check(lib.psx_create(ctypes.byref(params), ctypes.c_void_p(self.u0.data_ptr()),
+                                  ctypes.c_void_p(self.acc.data_ptr()),
+                                  ctypes.c_void_p(self.work.data_ptr()), ctypes.byref(ctx)))
+        self.ctx = ctx
+        self.lib = lib
+
+    def close(self) -> None:
+        if self.ctx:
+            import torch
+
+            torch.cuda.synchronize()
+            self.lib.psx_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+
+class HipSolver(SpectralSolver):
+    """Fused HIP kernels and the project's own FFTs (2D Basdevant; 3D Basdevant/rotational)."""
+
+    backend = "hip"
+    variant = "fused"
+    default_form = "basdevant"
+    forms = ("basdevant", "rotational")
+
+    def __init__(self, cfg: Config, graph: bool = False) -> None:
+        super().__init__(cfg)
+        if self.dim == 2 and self.form != "basdevant":
+            raise BackendUnavailable("hip: 2D supports the basdevant form only")
+        ok, why = self.available()
+        if not ok:
+            raise BackendUnavailable(f"hip: {why}")
+        import torch
+
+        self.torch = torch
+        self.cdt = torch.complex128 if cfg.is_fp64 else torch.complex64
+        self.params = flow_params(cfg, self.form)
+        self.buf = _Buffers(self.params, self.state_shape(), self.cdt)
+        self.lib = self.buf.lib
+        self.graph = graph
+        self._key: tuple[float, float] | None = None
+        self._ops = TorchOps()
+
+    @classmethod
+    def available(cls) -> tuple[bool, str]:
+        try:
+            import torch
+
+            if not torch.cuda.is_available():
+                return False, "no HIP device visible to torch"
+            _lib.load()
+            return _lib.device_info()
+        except Exception as e:  # noqa: BLE001
+            return False, f"{type(e).__name__}: {e}"
+
+    @property
+    def ops(self) -> Any:
+        return self._ops
+
+    @property
+    def ctx(self) -> ctypes.c_void_p:
+        return self.buf.ctx
+
+    # ------------------------------------------------------------------ state
+    def set_state(self, state_hat: np.ndarray) -> None:
+        """cupy_cuda.py:134-145: mask on the host, copy in, compute the first stage inputs."""
+        a = self.check_state(state_hat)
+        host = self.torch.from_numpy(a.astype(np.complex128 if self.cfg.is_fp64
+                                              else np.complex64))
+        self.buf.u0.copy_(host.to(self.buf.u0.device))
+        self.buf.acc.zero_()
+        _lib.check(self.lib.psx_prepare(self.ctx, _lib.stream_ptr()))
+
+    def set_state_device(self, state: Any) -> None:
+        """Load a device tensor already in NumPy layout and masked (no host round trip)."""
+        if tuple(state.shape) != self.state_shape():
+            raise ValueError(f"state shape {tuple(state.shape)} != {self.state_shape()}")
+        self.buf.u0.copy_(state.to(self.cdt))
+        self._mask_device(self.buf.u0)
+        _lib.check(self.lib.psx_prepare(self.ctx, _lib.stream_ptr()))
+
+    def _mask_device(self, t: Any) -> None:
+        n, jm = self.n, retained(self.n)
+        lo, hi = jm, n - jm + 1  # full-axis indices jm .. n-jm are outside 3|k| < n
+        t[..., jm:] = 0
+        if self.dim == 2:
+            t[lo:hi, :] = 0
+        else:
+            t[:, lo:hi, :, :] = 0
+            t[:, :, lo:hi, :] = 0
+
+    def state_native(self) -> Any:
+        return self.buf.u0
+
+    # ------------------------------------------------------------------ stepping
+    def _step(self, dt: float) -> None:
+        st = _lib.stream_ptr()
+        key = (float(dt), float(self.force_amp))
+        if key != self._key:
+            _lib.check(self.lib.psx_set_step(self.ctx, key[0], key[1], st))
+            self._key = key
+        if self.graph:
+            _lib.check(self.lib.psx_step_graph(self.ctx, st))
+        else:
+            _lib.check(self.lib.psx_step(self.ctx, st))
+
+    def _make_diag(self) -> Any:
+        from pyspectral_amd.diagnostics import DeviceDiagnostics
+
+        return DeviceDiagnostics(self)
+
+    def close(self) -> None:
+        self.buf.close()
+
+
+class HipGraphSolver(HipSolver):
+    """Same kernels, one step replayed from a hipGraph captured once."""
+
+    variant = "fused-graph"
+
+    def __init__(self, cfg: Config) -> None:
+        super().__init__(cfg, graph=True)

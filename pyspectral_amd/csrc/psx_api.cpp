@@ -1,0 +1,504 @@
+// Host side of libpsx: planner, context, C ABI (include/psx.h).
+#include "../../include/psx.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "psx_kernels.h"
+
+using namespace psx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return 1;
+}
+
+#define PSX_HIP(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) return fail("%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                                      __FILE__, __LINE__);                              \
+  } while (0)
+
+// ------------------------------------------------------------------------------- planning
+struct Factors {
+  int a = 0, b3 = 0, b5 = 0;
+  bool ok = false;
+};
+
+Factors factor(int L) {
+  Factors f;
+  int l = L;
+  while (l % 2 == 0) { l /= 2; ++f.a; }
+  while (l % 3 == 0) { l /= 3; ++f.b3; }
+  while (l % 5 == 0) { l /= 5; ++f.b5; }
+  f.ok = (l == 1);
+  return f;
+}
+
+// Radix list for length L with every radix dividing E; balanced power-of-two radices (<= 32)
+// first, then 3s and 5s.
+bool plan_fft(int L, int E, FftPlan* p) {
+  Factors f = factor(L);
+  if (!f.ok) return false;
+  if (f.b3 && E % 3) return false;
+  if (f.b5 && E % 5) return false;
+  int lmax = 0;
+  while (lmax < 5 && E % (1 << (lmax + 1)) == 0) ++lmax;
+  if (f.a > 0 && lmax == 0) return false;
+  const int np2 = f.a == 0 ? 0 : (f.a + lmax - 1) / lmax;
+  const int np = np2 + f.b3 + f.b5;
+  if (np > kMaxPasses || np == 0) return false;
+  p->L = L;
+  p->np = np;
+  int k = 0;
+  for (int i = 0; i < np2; ++i) {
+    const int bits = f.a / np2 + (i < f.a % np2 ? 1 : 0);
+    p->rad[k++] = 1 << bits;
+  }
+  for (int i = 0; i < f.b3; ++i) p->rad[k++] = 3;
+  for (int i = 0; i < f.b5; ++i) p->rad[k++] = 5;
+  for (; k < kMaxPasses; ++k) p->rad[k] = 1;
+  return true;
+}
+
+int pow2_floor(int x) {
+  int p = 1;
+  while (p * 2 <= x) p *= 2;
+  return p;
+}
+
+// Elements per thread for a transform kernel over length L in the given precision.
+int pick_E(int L, bool fp64, bool heavy) {
+  const bool r3 = (L % 3 == 0);
+  if (r3) return (fp64 || heavy) ? 12 : 24;
+  if (fp64) return heavy ? 8 : 16;
+  return 16;
+}
+
+// Row kernels: T whole rows of length L per work-group.
+bool plan_rows(int L, int nrows, bool fp64, bool heavy, int nmax_vals, Launch* lc, int* T,
+               FftPlan* p) {
+  int E = pick_E(L, fp64, heavy);
+  if (!plan_fft(L, E, p)) return false;
+  int t = 1;
+  while (L * t / E < 128 && t * 2 <= nrows) t *= 2;
+  int nt = L * t / E;
+  if (nt > 512) {
+    if (E == 16 && !fp64 && plan_fft(L, 32, p)) {
+      E = 32;
+      nt = L * t / E;
+    }
+    if (nt > 512) return false;
+  }
+  if ((L * t) % E) return false;
+  *T = t;
+  lc->E = E;
+  lc->threads = nt;
+  lc->blocks = (nrows + t - 1) / t;
+  const size_t csz = fp64 ? 16 : 8;
+  size_t lds = (size_t)t * rows_pitch(L) * csz;
+  const size_t red = (size_t)nmax_vals * nt * sizeof(double);
+  lc->lds = lds > red ? lds : red;
+  return true;
+}
+
+// Column kernels: tiles of L rows x T columns.
+bool plan_cols(int L, int T, int ngroups, int ldw, bool fp64, bool heavy, Launch* lc,
+               FftPlan* p) {
+  int E = pick_E(L, fp64, heavy);
+  while (E > 4 && (L * T) % E) E /= 2;
+  if (!plan_fft(L, E, p)) return false;
+  const int nt = L * T / E;
+  if (nt > 512 || nt < 1 || (L * T) % E) return false;
+  lc->E = E;
+  lc->threads = nt;
+  lc->blocks = ngroups * (ldw / T);
+  lc->lds = (size_t)L * T * (fp64 ? 16 : 8);
+  return true;
+}
+
+}  // namespace
+
+struct psx_ctx {
+  psx_flow_params p{};
+  psx_layout lay{};
+  bool fp64 = false;
+  Flow<float> f32{};
+  Flow<double> f64{};
+  Launch line{}, spec{}, outerF{}, outerI{}, yF{}, yI{};
+  void* tw = nullptr;
+  double* prm = nullptr;
+  unsigned long long* stats = nullptr;
+  double* partial = nullptr;
+  double* dout = nullptr;
+  int nred = 1024;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int nshell_max = 0;
+};
+
+namespace {
+
+bool supported_n(int n) {
+  if (n < 8 || n % 2) return false;
+  Factors f = factor(n);
+  return f.ok && f.b5 == 0 && f.b3 <= 1 && f.a >= 2;
+}
+
+// Fill the flow geometry and launch shapes (no device allocation).
+template <typename T>
+int plan_flow(const psx_flow_params& p, psx_ctx* c, Flow<T>* f) {
+  const bool fp64 = sizeof(T) == 8;
+  const int n = p.n;
+  if (!supported_n(n)) return fail("psx: n=%d unsupported (even, >= 8, 2^a or 3*2^a)", n);
+  if (p.dim != 2 && p.dim != 3) return fail("psx: dim must be 2 or 3");
+  if (p.dim == 2 && p.form != 0) return fail("psx: 2D supports the basdevant form only");
+  if (p.form != 0 && p.form != 1) return fail("psx: form must be 0 or 1");
+  std::memset(f, 0, sizeof *f);
+  f->n = n;
+  f->nh = n / 2 + 1;
+  f->jm = (n - 1) / 3 + 1;
+  f->dim = p.dim;
+  f->rot = p.dim == 3 && p.form == 1;
+  f->scale = (T)(1.0 / std::pow((double)n, p.dim));
+  const int colT = fp64 ? 8 : 16;
+  const int nrows = p.dim == 2 ? n : n * n;
+  const size_t csz = sizeof(T) * 2;
+  int T_ = colT;
+  while (T_ > 1 && T_ / 2 >= f->jm) T_ /= 2;  // tiny grids: do not pad far beyond jm
+  f->colT = T_;
+  f->yT = T_;
+  f->ldw = (f->jm + T_ - 1) / T_ * T_;
+  int nfields = 2;
+  if (p.dim == 3) nfields = f->rot ? 6 : 5;
+  f->fstride = (long long)nrows * f->ldw;
+  f->sstride = (long long)nrows * f->nh;
+
+  const int nmax = p.dim == 2 ? 2 : 3;
+  if (!plan_rows(n, nrows, fp64, p.dim == 3, nmax, &c->line, &f->rowT, &f->prow))
+    return fail("psx: no line plan for n=%d", n);
+  if (p.dim == 2) {
+    const size_t budget = 65536;
+    if ((size_t)n * T_ * csz <= budget || n <= 16) {
+      f->n1 = 1;
+      f->n2 = n;
+      if (!plan_cols(n, T_, 1, f->ldw, fp64, false, &c->spec, &f->pcol))
+        return fail("psx: no column plan for n=%d", n);
+    } else {
+      // n = n1 * n2, n2 >= n1, n2 the smallest divisor >= sqrt(n) that fits the budget
+      int best = 0;
+      for (int d = 2; d < n; ++d) {
+        if (n % d) continue;
+        if ((long long)d * d < n) continue;
+        if ((size_t)d * T_ * csz > budget) continue;
+        best = d;
+        break;
+      }
+      if (!best) return fail("psx: no four-step split for n=%d", n);
+      f->n2 = best;
+      f->n1 = n / best;
+      if (!plan_cols(f->n2, T_, f->n1, f->ldw, fp64, false, &c->spec, &f->pcol))
+        return fail("psx: no inner plan for n=%d", n);
+      if (!plan_cols(f->n1, T_, f->n2, f->ldw, fp64, false, &c->outerF, &f->pa))
+        return fail("psx: no outer plan for n=%d", n);
+      c->outerI = c->outerF;
+    }
+  } else {
+    if ((size_t)n * T_ * csz > 98304) return fail("psx: 3D n=%d too large for one tile", n);
+    f->n1 = 1;
+    f->n2 = n;
+    if (!plan_cols(n, T_, n, f->ldw, fp64, true, &c->spec, &f->pcol))
+      return fail("psx: no x plan for n=%d", n);
+    if (!plan_cols(n, T_, n, f->ldw, fp64, false, &c->yF, &f->py))
+      return fail("psx: no y plan for n=%d", n);
+    c->yI = c->yF;
+  }
+  c->lay.state_elems = (long long)(p.dim == 2 ? 1 : 3) * f->sstride;
+  c->lay.work_elems = (long long)nfields * f->fstride;
+  c->lay.nfields = nfields;
+  c->lay.ldw = f->ldw;
+  c->lay.jm = f->jm;
+  c->lay.four_step = f->n1 > 1;
+  return 0;
+}
+
+template <typename T>
+int make_twiddles(int n, void** out) {
+  std::vector<cx<T>> h(n);
+  for (int i = 0; i < n; ++i) {
+    const long double ang = 2.0L * 3.14159265358979323846264338327950288L * i / n;
+    h[i].x = (T)(double)std::cos(ang);
+    h[i].y = (T)(double)(-std::sin(ang));
+  }
+  PSX_HIP(hipMalloc(out, n * sizeof(cx<T>)));
+  PSX_HIP(hipMemcpy(*out, h.data(), n * sizeof(cx<T>), hipMemcpyHostToDevice));
+  return 0;
+}
+
+template <typename T>
+Flow<T>& flow(psx_ctx* c);
+template <>
+Flow<float>& flow<float>(psx_ctx* c) { return c->f32; }
+template <>
+Flow<double>& flow<double>(psx_ctx* c) { return c->f64; }
+
+template <typename T>
+int enqueue_prepare(psx_ctx* c, hipStream_t st) {
+  Flow<T>& f = flow<T>(c);
+  if (f.dim == 2) {
+    if (f.n1 > 1) {
+      PSX_HIP(launch_spec2d<T>(f, c->spec, true, kStagePre, st));
+      PSX_HIP(launch_outer2d<T>(f, c->outerI, true, st));
+    } else {
+      PSX_HIP(launch_spec2d<T>(f, c->spec, false, kStagePre, st));
+    }
+  } else {
+    PSX_HIP(launch_spec3d<T>(f, c->spec, kStagePre, st));
+    PSX_HIP(launch_ypass3d<T>(f, c->yI, true, f.rot ? 6 : 3, st));
+  }
+  return 0;
+}
+
+template <typename T>
+int enqueue_step(psx_ctx* c, hipStream_t st) {
+  Flow<T>& f = flow<T>(c);
+  for (int s = 0; s < 4; ++s) {
+    if (f.dim == 2) {
+      PSX_HIP(launch_line2d<T>(f, c->line, 0, st));
+      if (f.n1 > 1) {
+        PSX_HIP(launch_outer2d<T>(f, c->outerF, false, st));
+        PSX_HIP(launch_spec2d<T>(f, c->spec, true, s, st));
+        PSX_HIP(launch_outer2d<T>(f, c->outerI, true, st));
+      } else {
+        PSX_HIP(launch_spec2d<T>(f, c->spec, false, s, st));
+      }
+    } else {
+      PSX_HIP(launch_line3d<T>(f, c->line, 0, st));
+      PSX_HIP(launch_ypass3d<T>(f, c->yF, false, f.rot ? 3 : 5, st));
+      PSX_HIP(launch_spec3d<T>(f, c->spec, s, st));
+      PSX_HIP(launch_ypass3d<T>(f, c->yI, true, f.rot ? 6 : 3, st));
+    }
+  }
+  return 0;
+}
+
+template <typename T>
+int velocity_max(psx_ctx* c, double* out, hipStream_t st) {
+  Flow<T>& f = flow<T>(c);
+  PSX_HIP(launch_fill_u32(reinterpret_cast<unsigned*>(c->stats), 0u, 6, st));
+  if (f.dim == 2) {
+    PSX_HIP(launch_line2d<T>(f, c->line, kLineWantMax | kLineMaxOnly, st));
+  } else {
+    PSX_HIP(launch_line3d<T>(f, c->line, kLineWantMax | kLineMaxOnly, st));
+  }
+  unsigned long long h[3] = {0, 0, 0};
+  PSX_HIP(hipMemcpyAsync(h, c->stats, sizeof h, hipMemcpyDeviceToHost, st));
+  PSX_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < 3; ++i) {
+    double v;
+    std::memcpy(&v, &h[i], sizeof v);
+    out[i] = v;
+  }
+  if (f.dim == 2) out[2] = 0.0;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* psx_last_error(void) { return g_err.c_str(); }
+
+int psx_device_info(char* buf, int len) {
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev < 1) {
+    if (buf && len > 0) snprintf(buf, len, "no HIP device");
+    return fail("psx: no HIP device");
+  }
+  int dev = 0;
+  PSX_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  PSX_HIP(hipGetDeviceProperties(&prop, dev));
+  if (buf && len > 0)
+    snprintf(buf, len, "%s %s lds/block=%zu", prop.name, prop.gcnArchName,
+             (size_t)prop.sharedMemPerBlock);
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail("psx: device %s is not gfx950", prop.gcnArchName);
+  return 0;
+}
+
+int psx_query_layout(const psx_flow_params* p, psx_layout* out) {
+  if (!p || !out) return fail("psx_query_layout: null argument");
+  psx_ctx tmp;
+  tmp.p = *p;
+  int r = p->fp64 ? plan_flow<double>(*p, &tmp, &tmp.f64) : plan_flow<float>(*p, &tmp, &tmp.f32);
+  if (r) return r;
+  *out = tmp.lay;
+  return 0;
+}
+
+int psx_create(const psx_flow_params* p, void* u0, void* acc, void* work, psx_ctx** out) {
+  if (!p || !u0 || !acc || !work || !out) return fail("psx_create: null argument");
+  psx_ctx* c = new psx_ctx;
+  c->p = *p;
+  c->fp64 = p->fp64 != 0;
+  int r = c->fp64 ? plan_flow<double>(*p, c, &c->f64) : plan_flow<float>(*p, c, &c->f32);
+  if (r) {
+    delete c;
+    return r;
+  }
+  const int n = p->n, nh = n / 2 + 1;
+  r = c->fp64 ? make_twiddles<double>(n, &c->tw) : make_twiddles<float>(n, &c->tw);
+  if (r) {
+    delete c;
+    return r;
+  }
+  const int nprm = 2 * nh + PrmLayout::kNScalars;
+  c->nshell_max = (int)std::floor(std::sqrt((double)p->dim) * (n / 2) + 0.5) + 1;
+  if (hipMalloc(&c->prm, nprm * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->stats, 4 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&c->partial, (size_t)c->nred * 4 * sizeof(double)) != hipSuccess ||
+      hipMalloc(&c->dout, (size_t)(c->nshell_max + 8) * sizeof(double)) != hipSuccess) {
+    psx_destroy(c);
+    return fail("psx_create: hipMalloc failed");
+  }
+  // default parameters: dt = 0, no forcing (the caller sets them before stepping)
+  hipStream_t st = nullptr;
+  if (launch_params(c->prm, nh, p->nu, 0.0, p->alpha, 0.0, p->klo2, p->khi2, p->ka2, st) !=
+          hipSuccess ||
+      hipStreamSynchronize(st) != hipSuccess) {
+    psx_destroy(c);
+    return fail("psx_create: parameter kernel failed");
+  }
+  auto fill = [&](auto& f) {
+    using T = std::remove_reference_t<decltype(f.scale)>;
+    f.W = static_cast<cx<T>*>(work);
+    f.u0 = static_cast<cx<T>*>(u0);
+    f.acc = static_cast<cx<T>*>(acc);
+    f.tw = static_cast<const cx<T>*>(c->tw);
+    f.prm = c->prm;
+    f.stats = c->stats;
+  };
+  if (c->fp64) fill(c->f64);
+  else fill(c->f32);
+  *out = c;
+  return 0;
+}
+
+int psx_destroy(psx_ctx* c) {
+  if (!c) return 0;
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->cap) (void)hipStreamDestroy(c->cap);
+  if (c->tw) (void)hipFree(c->tw);
+  if (c->prm) (void)hipFree(c->prm);
+  if (c->stats) (void)hipFree(c->stats);
+  if (c->partial) (void)hipFree(c->partial);
+  if (c->dout) (void)hipFree(c->dout);
+  delete c;
+  return 0;
+}
+
+int psx_prepare(psx_ctx* c, void* stream) {
+  if (!c) return fail("psx_prepare: null context");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return c->fp64 ? enqueue_prepare<double>(c, st) : enqueue_prepare<float>(c, st);
+}
+
+int psx_set_step(psx_ctx* c, double dt, double amp, void* stream) {
+  if (!c) return fail("psx_set_step: null context");
+  if (!std::isfinite(dt) || !std::isfinite(amp)) return fail("psx_set_step: non-finite dt/amp");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int nh = c->p.n / 2 + 1;
+  PSX_HIP(launch_params(c->prm, nh, c->p.nu, dt, c->p.alpha, amp, c->p.klo2, c->p.khi2,
+                        c->p.ka2, st));
+  return 0;
+}
+
+int psx_step(psx_ctx* c, void* stream) {
+  if (!c) return fail("psx_step: null context");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return c->fp64 ? enqueue_step<double>(c, st) : enqueue_step<float>(c, st);
+}
+
+int psx_step_graph(psx_ctx* c, void* stream) {
+  if (!c) return fail("psx_step_graph: null context");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (!c->gexec) {
+    if (!c->cap) PSX_HIP(hipStreamCreateWithFlags(&c->cap, hipStreamNonBlocking));
+    PSX_HIP(hipStreamBeginCapture(c->cap, hipStreamCaptureModeRelaxed));
+    int r = c->fp64 ? enqueue_step<double>(c, c->cap) : enqueue_step<float>(c, c->cap);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->cap, &g);
+    if (r) {
+      if (g) (void)hipGraphDestroy(g);
+      return r;
+    }
+    if (e != hipSuccess) return fail("psx_step_graph: capture failed: %s", hipGetErrorString(e));
+    c->graph = g;
+    PSX_HIP(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+  }
+  PSX_HIP(hipGraphLaunch(c->gexec, st));
+  return 0;
+}
+
+int psx_velocity_max(psx_ctx* c, double* out3, void* stream) {
+  if (!c || !out3) return fail("psx_velocity_max: null argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  return c->fp64 ? velocity_max<double>(c, out3, st) : velocity_max<float>(c, out3, st);
+}
+
+int psx_reduce(psx_ctx* c, const void* state, double klo2, double khi2, double* out4,
+               void* stream) {
+  if (!c || !state || !out4) return fail("psx_reduce: null argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int n = c->p.n, dim = c->p.dim;
+  const long long sst = (long long)(dim == 2 ? n : n * n) * (n / 2 + 1);
+  if (c->fp64)
+    PSX_HIP(launch_reduce<double>(static_cast<const cx<double>*>(state), sst, n, dim, klo2,
+                                  khi2, c->partial, c->nred, c->dout, st));
+  else
+    PSX_HIP(launch_reduce<float>(static_cast<const cx<float>*>(state), sst, n, dim, klo2, khi2,
+                                 c->partial, c->nred, c->dout, st));
+  PSX_HIP(hipMemcpyAsync(out4, c->dout, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+  PSX_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+int psx_spectrum(psx_ctx* c, const void* state, double* out, int nshell, void* stream) {
+  if (!c || !state || !out) return fail("psx_spectrum: null argument");
+  if (nshell < 1 || nshell > c->nshell_max)
+    return fail("psx_spectrum: nshell=%d outside [1, %d]", nshell, c->nshell_max);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int n = c->p.n, dim = c->p.dim;
+  const long long sst = (long long)(dim == 2 ? n : n * n) * (n / 2 + 1);
+  if (c->fp64)
+    PSX_HIP(launch_spectrum<double>(static_cast<const cx<double>*>(state), sst, n, dim,
+                                    c->dout, nshell, st));
+  else
+    PSX_HIP(launch_spectrum<float>(static_cast<const cx<float>*>(state), sst, n, dim, c->dout,
+                                   nshell, st));
+  PSX_HIP(hipMemcpyAsync(out, c->dout, nshell * sizeof(double), hipMemcpyDeviceToHost, st));
+  PSX_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+}  // extern "C"

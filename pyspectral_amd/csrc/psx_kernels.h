@@ -1,0 +1,59 @@
+// Declarations shared by the kernel translation units and the host planner.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "psx_stage.h"
+
+namespace psx {
+
+// Launch shape chosen by the planner for one kernel.
+struct Launch {
+  int E;        // elements per thread (compile-time variant)
+  int threads;  // work-group size
+  int blocks;   // grid size
+  size_t lds;   // dynamic LDS bytes
+};
+
+enum : int { kLineWantMax = 1, kLineMaxOnly = 2 };
+
+// Allow a kernel the whole LDS of a gfx950 CU (160 KB) as dynamic shared memory; called once
+// per kernel from the launchers.
+inline bool set_lds_max(const void* fn) {
+  return hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) ==
+         hipSuccess;
+}
+
+// Elements-per-thread variants compiled for every transform kernel.
+#define PSX_E_CASES PSX_CASE(8) PSX_CASE(12) PSX_CASE(16) PSX_CASE(24) PSX_CASE(32)
+
+template <typename T>
+hipError_t launch_line2d(const Flow<T>& f, const Launch& lc, int flags, hipStream_t st);
+template <typename T>
+hipError_t launch_spec2d(const Flow<T>& f, const Launch& lc, bool four, int stage,
+                         hipStream_t st);
+template <typename T>
+hipError_t launch_outer2d(const Flow<T>& f, const Launch& lc, bool inv, hipStream_t st);
+
+template <typename T>
+hipError_t launch_line3d(const Flow<T>& f, const Launch& lc, int flags, hipStream_t st);
+template <typename T>
+hipError_t launch_ypass3d(const Flow<T>& f, const Launch& lc, bool inv, int nfld,
+                          hipStream_t st);
+template <typename T>
+hipError_t launch_spec3d(const Flow<T>& f, const Launch& lc, int stage, hipStream_t st);
+
+// Utility / diagnostics kernels (psx_kutil.hip).
+hipError_t launch_params(double* prm, int nh, double nu, double dt, double alpha, double amp,
+                         double klo2, double khi2, double ka2, hipStream_t st);
+hipError_t launch_fill_u32(unsigned* p, unsigned v, int n, hipStream_t st);
+hipError_t launch_fill_f64(double* p, double v, int n, hipStream_t st);
+template <typename T>
+hipError_t launch_reduce(const cx<T>* state, long long sstride, int n, int dim, double klo2,
+                         double khi2, double* partial, int nblocks, double* out,
+                         hipStream_t st);
+template <typename T>
+hipError_t launch_spectrum(const cx<T>* state, long long sstride, int n, int dim,
+                           double* out, int nshell, hipStream_t st);
+
+}  // namespace psx

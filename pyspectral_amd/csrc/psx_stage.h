@@ -1,0 +1,117 @@
+// Pointwise physics shared by the 2D and 3D stage kernels: integrating-factor values and the
+// Lawson IF-RK4 stage algebra.
+//
+// Reference semantics: src/pyspectral/kernels/spectral.cu:43-80 (if_factors, rk_stage) and
+// src/pyspectral/backends/numpy_ref.py:145-153 (the Lawson combination).  Unlike the
+// reference, whose table is rounded to the working precision on the host
+// (cupy_cuda.py:151-161), the factors are formed here in FP64 from an FP64 table that the
+// parameter kernel computes on the device, and rounded once.
+#pragma once
+
+#include "psx_flow.h"
+
+namespace psx {
+
+enum { kStagePre = -1 };
+
+struct IfScalars {
+  double ea1, eah, eb1, ebh, klo2, khi2, ka2;
+  bool band;
+};
+
+__device__ __forceinline__ IfScalars load_if_scalars(const double* __restrict__ prm, int nh) {
+  const double* s = prm + 2 * nh;
+  IfScalars r;
+  r.ea1 = s[PrmLayout::kEa1];
+  r.eah = s[PrmLayout::kEah];
+  r.eb1 = s[PrmLayout::kEb1];
+  r.ebh = s[PrmLayout::kEbh];
+  r.klo2 = s[PrmLayout::kKlo2];
+  r.khi2 = s[PrmLayout::kKhi2];
+  r.ka2 = s[PrmLayout::kKa2];
+  r.band = s[PrmLayout::kBand] != 0.0;
+  return r;
+}
+
+// exp(L dt) and exp(L dt/2) for one mode: separable viscous part from the tables, times the
+// drag factor when k^2 <= ka^2 and the band factor when klo^2 <= k^2 <= khi^2.
+// az < 0 for 2D.
+__device__ __forceinline__ void if_factors(const double* __restrict__ prm, int nh,
+                                           const IfScalars& s, int ax, int ay, int az,
+                                           long long k2, double& E, double& Eh) {
+  E = prm[ax] * prm[ay];
+  Eh = prm[nh + ax] * prm[nh + ay];
+  if (az >= 0) {
+    E *= prm[az];
+    Eh *= prm[nh + az];
+  }
+  const double dk2 = (double)k2;
+  if (dk2 <= s.ka2) {
+    E *= s.ea1;
+    Eh *= s.eah;
+  }
+  if (s.band && dk2 >= s.klo2 && dk2 <= s.khi2) {
+    E *= s.eb1;
+    Eh *= s.ebh;
+  }
+}
+
+// One Lawson stage for one mode/component.  With k1..k4 the stage nonlinear terms:
+//   stage 0: acc = E k1,            next = Eh (u + dt/2 k1)
+//   stage 1: acc += 2 Eh k2,        next = Eh u + dt/2 k2
+//   stage 2: acc += 2 Eh k3,        next = E u + dt Eh k3
+//   stage 3: u = E u + dt/6 (acc + k4), next = u
+template <typename T>
+__device__ __forceinline__ cx<T> rk_update(int stage, T dt, T E, T Eh, cx<T> N,
+                                           cx<T>* __restrict__ up, cx<T>* __restrict__ ap) {
+  const cx<T> u = *up;
+  if (stage == 0) {
+    *ap = cscl(E, N);
+    return cscl(Eh, cadd(u, cscl((T)0.5 * dt, N)));
+  }
+  if (stage == 1) {
+    *ap = cadd(*ap, cscl((T)2 * Eh, N));
+    return cadd(cscl(Eh, u), cscl((T)0.5 * dt, N));
+  }
+  if (stage == 2) {
+    *ap = cadd(*ap, cscl((T)2 * Eh, N));
+    return cadd(cscl(E, u), cscl(dt * Eh, N));
+  }
+  const cx<T> r = cadd(cscl(E, u), cscl(dt / (T)6, cadd(*ap, N)));
+  *up = r;
+  return r;
+}
+
+// Block-wide max of NV non-negative values, folded into stats[] with one atomic per
+// work-group (bit patterns of non-negative doubles order like unsigned 64-bit integers).
+// Works for any block size up to 512, including partial wavefronts.  `red` is LDS scratch of
+// at least NV * blockDim.x doubles that no thread still reads (caller synchronises before).
+template <int NV>
+__device__ __forceinline__ void block_max_to(double (&m)[NV],
+                                             unsigned long long* __restrict__ stats,
+                                             double* red) {
+  const int tid = threadIdx.x, nt = blockDim.x;
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < NV; ++c) red[c * nt + tid] = m[c];
+  __syncthreads();
+  for (int s = 256; s > 0; s >>= 1) {
+    if (tid < s && tid + s < nt) {
+#pragma unroll
+      for (int c = 0; c < NV; ++c) red[c * nt + tid] = fmax(red[c * nt + tid], red[c * nt + tid + s]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+#pragma unroll
+    for (int c = 0; c < NV; ++c)
+      atomicMax(stats + c, (unsigned long long)__double_as_longlong(red[c * nt]));
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ double abs_or_inf(double a) {
+  return __builtin_isnan(a) ? __builtin_inf() : fabs(a);
+}
+
+}  // namespace psx

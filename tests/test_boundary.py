@@ -1,0 +1,119 @@
+"""The drop-in boundary on the CPU: the C ABI library, the registry and the host-side mirror of
+the reference interface (no compute calls; those need a GPU)."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import numpy_oracle as O
+from pyspectral_amd import _lib, grid, initial
+from pyspectral_amd.backends import REGISTRY, BackendUnavailable, get, names
+from pyspectral_amd.config import Config, fft_split, ffts_per_stage
+
+ROOT = Path(__file__).resolve().parents[1]
+GOLD = ROOT / "tests" / "golden"
+
+
+def _header_symbols() -> list[str]:
+    text = (ROOT / "include" / "psx.h").read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(psx_\w+)\s*\(", text, re.M)))
+
+
+def test_header_and_binding_agree():
+    assert set(_header_symbols()) == set(_lib.SIGNATURES)
+
+
+@pytest.mark.skipif(not _lib.LIB_PATH.exists(), reason="libpsx.so not built")
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    for name in _header_symbols():
+        assert hasattr(lib, name), name
+
+
+@pytest.mark.skipif(not _lib.LIB_PATH.exists(), reason="libpsx.so not built")
+@pytest.mark.parametrize("n,dim,form,fp64,ok", [
+    (8192, 2, 0, 0, True), (256, 2, 0, 1, True), (32, 2, 0, 0, True), (16, 3, 1, 1, True),
+    (384, 3, 0, 0, True), (96, 2, 0, 0, True), (10, 2, 0, 0, False), (32, 2, 1, 0, False),
+    (7, 2, 0, 0, False)])
+def test_layout_query(n, dim, form, fp64, ok):
+    lib = _lib.load()
+    p = _lib.FlowParams(n=n, dim=dim, form=form, fp64=fp64, nu=1e-3, alpha=0.0, ka2=1e30,
+                        klo2=0.0, khi2=0.0)
+    lay = _lib.Layout()
+    rc = lib.psx_query_layout(ctypes.byref(p), ctypes.byref(lay))
+    assert (rc == 0) == ok, lib.psx_last_error()
+    if ok:
+        nh = n // 2 + 1
+        rows = n ** (dim - 1)
+        assert lay.state_elems == (1 if dim == 2 else 3) * rows * nh
+        assert lay.jm == grid.retained(n) and lay.ldw >= lay.jm
+        assert lay.nfields == (2 if dim == 2 else (6 if form else 5))
+        assert lay.work_elems == lay.nfields * rows * lay.ldw
+    else:
+        assert lib.psx_last_error()
+
+
+def test_registry_surface():
+    assert names() == ["hip/fused", "hip/fused-graph"]
+    assert get("hip").variant == "fused"
+    with pytest.raises(KeyError):
+        get("nope")
+    for v in REGISTRY:
+        ok, why = v.available()
+        assert isinstance(ok, bool) and isinstance(why, str)
+
+
+def test_unavailable_without_gpu_raises_backend_unavailable():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(BackendUnavailable):
+        get("hip/fused").make(Config(n=16, dim=2))
+
+
+def test_config_mirrors_reference():
+    assert ffts_per_stage(2, "basdevant") == 4 and ffts_per_stage(3, "rotational") == 9
+    assert fft_split(3, "basdevant") == (3, 5)
+    assert Config(n=16, dim=3).resolved_form() == "basdevant"
+    assert Config(n=16, alpha_kmax=2.0).drag_k2 == 4.0 and Config(n=16).drag_k2 == 1e30
+    for bad in ({"n": 7}, {"n": 16, "dim": 4}, {"n": 16, "dtype": "float16"},
+                {"n": 16, "form": "rotational"}, {"n": 16, "dealias": "x"}):
+        with pytest.raises(ValueError):
+            Config(**bad)
+
+
+def test_grid_matches_oracle():
+    for n, d in ((12, 2), (16, 3), (18, 2)):
+        assert grid.spectral_shape(n, d) == O.spectral_shape(n, d)
+        np.testing.assert_array_equal(grid.dealias_mask(n, d), O.dealias_mask(n, d))
+        np.testing.assert_array_equal(grid.k_squared(n, d), O.k_squared(n, d))
+        assert grid.retained(n) == int(O.dealias_mask(n, d).reshape(-1, n // 2 + 1)[0].sum())
+
+
+def test_initial_conditions_match_reference_vectors():
+    with np.load(GOLD / "ic.npz", allow_pickle=False) as g:
+        np.testing.assert_allclose(initial.random_2d(32, k0=4), g["random_2d_32_k4"], atol=1e-13)
+        np.testing.assert_allclose(initial.random_3d(16, k0=2), g["random_3d_16_k2"], atol=1e-13)
+        np.testing.assert_allclose(initial.smooth_2d(32), g["smooth_2d_32"], atol=1e-13)
+
+
+def test_random_phase_ic_spectrum_and_energy():
+    n, k0 = 64, 6.0
+    w = initial.random_phase_2d(n, k0=k0, seed=3, energy=0.5)
+    d = O.OracleDiagnostics(n, 2)
+    assert d.energy(w) == pytest.approx(0.5, rel=1e-12)
+    assert np.all(w[~grid.dealias_mask(n, 2)] == 0)
+    # Hermitian ky = 0 column -> the physical field is real (irfft round trip is exact)
+    f = np.fft.irfftn(w, s=(n, n))
+    np.testing.assert_allclose(np.fft.rfftn(f), w, atol=1e-9 * np.abs(w).max())
+    k, e = d.spectrum(w)
+    sel = (k >= 3) & (k <= 12)
+    model = k[sel] ** 4 * np.exp(-((k[sel] / k0) ** 2))
+    ratio = e[sel] / model
+    assert ratio.std() / ratio.mean() < 0.35  # shell sums follow k^4 exp(-(k/k0)^2)

@@ -1,0 +1,164 @@
+"""HIP path against the CPU oracle on identical inputs (GPU only).
+
+Protocols follow the reference's own GPU tests (tests/test_backends.py:24-73): 5 steps of
+dt=0.01, a forcing-amplitude change, 3 steps of dt=0.007; tolerances max|a-b|/max|b| below
+1e-12 (fp64) and 5e-6 (fp32).  Larger grids exercise the four-step path and the radix-3
+plans; full-size runs are checked through exact solutions and invariants.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import numpy_oracle as O
+from pyspectral_amd import initial
+from pyspectral_amd.backends import BackendUnavailable, get
+from pyspectral_amd.config import Config
+
+pytestmark = pytest.mark.gpu
+TOL = {"float64": 1e-12, "float32": 5e-6}
+VARIANTS = ["hip/fused", "hip/fused-graph"]
+
+
+def _make(name, cfg):
+    try:
+        return get(name).make(cfg)
+    except BackendUnavailable as e:  # a GPU box without the library must fail, not skip
+        raise AssertionError(f"{name} unavailable on a GPU box: {e}") from e
+
+
+def _ic(n, dim, k0=None, seed=0):
+    if dim == 2:
+        return initial.to_spectral(initial.random_2d(n, k0=k0 or 4, seed=seed), 2)
+    return initial.to_spectral(initial.random_3d(n, k0=k0 or 2, seed=seed), 3)
+
+
+@pytest.mark.parametrize("name", VARIANTS)
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+@pytest.mark.parametrize("dim,form", [(2, "basdevant"), (3, "basdevant"), (3, "rotational")])
+@pytest.mark.parametrize("alpha_kmax", [0.0, 2.0])
+def test_matches_oracle(name, dtype, dim, form, alpha_kmax):
+    n = 32 if dim == 2 else 16
+    print(f"case {name} {dtype} {dim}D {form} akm={alpha_kmax} n={n}", flush=True)
+    cfg = Config(n=n, dim=dim, nu=0.01, alpha=0.5, alpha_kmax=alpha_kmax, force_amp=0.3,
+                 force_klo=1, force_khi=2.5, dtype=dtype, form=form)
+    s = _make(name, cfg)
+    ref = O.OracleSolver(cfg.with_(dtype="float64"))
+    ic = _ic(n, dim)
+    s.set_state(ic)
+    ref.set_state(ic)
+    s.advance(5, 0.01)
+    ref.advance(5, 0.01)
+    s.set_forcing_amplitude(0.1)
+    ref.set_forcing_amplitude(0.1)
+    s.advance(3, 0.007)
+    ref.advance(3, 0.007)
+    err = O.rel_err(s.get_state(), ref.get_state())
+    print(f"  rel err {err:.3e}", flush=True)
+    assert err < TOL[dtype], err
+
+
+@pytest.mark.parametrize("name", VARIANTS)
+def test_tgv2d_exact(name):
+    n, nu = 32, 0.05
+    s = _make(name, Config(n=n, dim=2, nu=nu, dtype="float64"))
+    w0 = initial.to_spectral(initial.taylor_green_2d(n), 2)
+    s.set_state(w0)
+    s.advance(20, 0.05)
+    assert O.rel_err(s.get_state(), w0 * math.exp(-2 * nu * 1.0)) < 1e-13
+
+
+@pytest.mark.parametrize("n,dtype,steps", [(96, "float64", 3), (128, "float32", 3),
+                                           (384, "float32", 2), (1024, "float32", 2),
+                                           (1024, "float64", 2), (2048, "float32", 1)])
+def test_2d_sizes_and_four_step(n, dtype, steps):
+    print(f"case 2D n={n} {dtype}", flush=True)
+    cfg = Config(n=n, dim=2, nu=1e-3, alpha=0.02, force_amp=0.2, force_klo=2, force_khi=4,
+                 dtype=dtype)
+    s = _make("hip/fused", cfg)
+    ref = O.OracleSolver(cfg.with_(dtype="float64"))
+    ic = initial.random_phase_2d(n, k0=min(10.0, n / 12), seed=1)
+    s.set_state(ic)
+    ref.set_state(ic)
+    dt = 0.2 * (2 * math.pi / n)
+    s.advance(steps, dt)
+    ref.advance(steps, dt)
+    err = O.rel_err(s.get_state(), ref.get_state())
+    print(f"  rel err {err:.3e}", flush=True)
+    assert err < TOL[dtype], err
+
+
+@pytest.mark.parametrize("n,form,dtype", [(48, "basdevant", "float32"),
+                                          (64, "rotational", "float32"),
+                                          (48, "rotational", "float64"),
+                                          (32, "basdevant", "float64")])
+def test_3d_sizes(n, form, dtype):
+    print(f"case 3D n={n} {form} {dtype}", flush=True)
+    cfg = Config(n=n, dim=3, nu=1 / 1600, dtype=dtype, form=form)
+    s = _make("hip/fused", cfg)
+    ref = O.OracleSolver(cfg.with_(dtype="float64"))
+    ic = initial.to_spectral(initial.taylor_green_3d(n), 3) + 0.01 * _ic(n, 3, k0=3, seed=4)
+    s.set_state(ic)
+    ref.set_state(ic)
+    s.advance(3, 0.02)
+    ref.advance(3, 0.02)
+    err = O.rel_err(s.get_state(), ref.get_state())
+    print(f"  rel err {err:.3e}", flush=True)
+    assert err < TOL[dtype], err
+
+
+@pytest.mark.parametrize("dim,n,dtype", [(2, 32, "float64"), (3, 16, "float64"),
+                                         (2, 64, "float32"), (3, 24, "float32")])
+def test_device_diagnostics_match_oracle(dim, n, dtype):
+    s = _make("hip/fused", Config(n=n, dim=dim, dtype=dtype))
+    st = _ic(n, dim, seed=7)
+    s.set_state(st)
+    ref = s.check_state(st)
+    if dtype == "float32":
+        ref = ref.astype(np.complex64).astype(np.complex128)
+    h = O.OracleDiagnostics(n, dim)
+    d = s.diag
+    sn = s.state_native()
+    rtol = 1e-12 if dtype == "float64" else 1e-10
+    assert d.energy(sn) == pytest.approx(h.energy(ref), rel=rtol)
+    assert d.enstrophy(sn) == pytest.approx(h.enstrophy(ref), rel=rtol)
+    assert d.palinstrophy(sn) == pytest.approx(h.palinstrophy(ref), rel=rtol)
+    assert d.band_energy(sn, 1.0, 2.5) == pytest.approx(h.band_energy(ref, 1.0, 2.5), rel=rtol)
+    np.testing.assert_allclose(d.spectrum(sn)[1], h.spectrum(ref)[1], rtol=rtol, atol=1e-300)
+    vtol = 1e-12 if dtype == "float64" else 2e-6
+    assert d.max_velocity_sum(sn) == pytest.approx(h.max_velocity_sum(ref), rel=vtol)
+    # a state that is not the solver's own goes through a scratch context
+    other = s.ops.asarray(ref * 0.5, dtype=s.cdt)
+    assert d.max_velocity_sum(other) == pytest.approx(0.5 * h.max_velocity_sum(ref), rel=vtol)
+
+
+def test_graph_replays_match_eager():
+    cfg = Config(n=64, dim=2, nu=2e-3, dtype="float32", force_amp=0.1, force_klo=2, force_khi=3)
+    a = _make("hip/fused", cfg)
+    b = _make("hip/fused-graph", cfg)
+    ic = _ic(64, 2, k0=6)
+    for s in (a, b):
+        s.set_state(ic)
+    for k in range(6):  # eager work and syncs between replays, dt and amp changing
+        dt = 0.01 if k < 3 else 0.006
+        for s in (a, b):
+            s.set_forcing_amplitude(0.1 * (k % 2))
+            s.step(dt)
+        a.synchronize()
+        assert np.array_equal(a.get_state(), b.get_state()), k
+
+
+def test_tgv2d_full_size_exact_decay():
+    """8192^2 fp32: the Taylor-Green vortex is an exact solution at every n (size-independent
+    property of the full-size grid)."""
+    n, nu = 8192, 1e-2
+    s = _make("hip/fused-graph", Config(n=n, dim=2, nu=nu, dtype="float32"))
+    w0 = initial.to_spectral(initial.taylor_green_2d(n), 2)
+    s.set_state(w0)
+    s.advance(5, 1e-3)
+    got = s.get_state()
+    exact = w0 * math.exp(-2 * nu * s.t)
+    assert O.rel_err(got, exact) < 5e-6
