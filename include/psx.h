@@ -84,6 +84,23 @@ int psx_step(psx_ctx* c, void* stream);
  * memory, so the graph is captured once).  Replaces cupy_cuda.py:181-195 (graph variant). */
 int psx_step_graph(psx_ctx* c, void* stream);
 
+/* Kernel kinds reported by psx_timing_end. */
+#define PSX_KIND_LINE 0      /* real-axis line kernel: c2r, products, r2c */
+#define PSX_KIND_OUTER_FWD 1 /* 2D four-step outer forward pass */
+#define PSX_KIND_SPEC 2      /* spectral kernel: forward, physics, inverse along the outer axis */
+#define PSX_KIND_OUTER_INV 3 /* 2D four-step outer inverse pass */
+#define PSX_KIND_MID_FWD 4   /* 3D middle-axis forward pass */
+#define PSX_KIND_MID_INV 5   /* 3D middle-axis inverse pass */
+#define PSX_NKINDS 6
+
+/* Per-launch GPU timing of eager steps with HIP events on the caller's stream: between
+ * psx_timing_begin and psx_timing_end every psx_step records an event before each launch;
+ * psx_timing_end synchronises and returns, per kernel kind, the summed milliseconds and the
+ * launch count.  No reference counterpart (the reference times whole steps with wall clocks,
+ * src/pyspectral/benchmark.py:120-142); used by the benchmark's roofline figures. */
+int psx_timing_begin(psx_ctx* c);
+int psx_timing_end(psx_ctx* c, void* stream, double* ms, int* count);
+
 /* max |u_i| over the physical grid of the current state, per component (2 in 2D, 3 in 3D);
  * synchronises the stream.  Replaces src/pyspectral/diagnostics.py:99-109
  * (max_velocity_sum, which sums these). */

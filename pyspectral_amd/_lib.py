@@ -38,9 +38,13 @@ SIGNATURES = {
     "psx_reduce": (c_int, [c_void_p, c_void_p, c_double, c_double, POINTER(c_double),
                            c_void_p]),
     "psx_spectrum": (c_int, [c_void_p, c_void_p, POINTER(c_double), c_int, c_void_p]),
+    "psx_timing_begin": (c_int, [c_void_p]),
+    "psx_timing_end": (c_int, [c_void_p, c_void_p, POINTER(c_double), POINTER(c_int)]),
     "psx_last_error": (c_char_p, []),
     "psx_device_info": (c_int, [c_char_p, c_int]),
 }
+
+KINDS = ("line", "outer_fwd", "spec", "outer_inv", "mid_fwd", "mid_inv")  # PSX_KIND_*
 
 _LIB: ctypes.CDLL | None = None
 

@@ -152,6 +152,11 @@ struct psx_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int nshell_max = 0;
+  // per-launch event timing of eager steps (psx_timing_*)
+  bool timing = false;
+  std::vector<hipEvent_t> events;
+  std::vector<int> kinds;
+  int nrec = 0;
 };
 
 namespace {
@@ -259,6 +264,27 @@ Flow<float>& flow<float>(psx_ctx* c) { return c->f32; }
 template <>
 Flow<double>& flow<double>(psx_ctx* c) { return c->f64; }
 
+// Record a timing event before a launch of the given kind (-1: end of a step).
+int mark(psx_ctx* c, int kind, hipStream_t st) {
+  if (!c->timing) return 0;
+  if (c->nrec == (int)c->events.size()) {
+    hipEvent_t e;
+    PSX_HIP(hipEventCreate(&e));
+    c->events.push_back(e);
+    c->kinds.push_back(-1);
+  }
+  PSX_HIP(hipEventRecord(c->events[c->nrec], st));
+  c->kinds[c->nrec] = kind;
+  ++c->nrec;
+  return 0;
+}
+
+#define PSX_MARK(kind) \
+  do {                 \
+    int r_ = mark(c, (kind), st); \
+    if (r_) return r_; \
+  } while (0)
+
 template <typename T>
 int enqueue_prepare(psx_ctx* c, hipStream_t st) {
   Flow<T>& f = flow<T>(c);
@@ -281,21 +307,31 @@ int enqueue_step(psx_ctx* c, hipStream_t st) {
   Flow<T>& f = flow<T>(c);
   for (int s = 0; s < 4; ++s) {
     if (f.dim == 2) {
+      PSX_MARK(PSX_KIND_LINE);
       PSX_HIP(launch_line2d<T>(f, c->line, 0, st));
       if (f.n1 > 1) {
+        PSX_MARK(PSX_KIND_OUTER_FWD);
         PSX_HIP(launch_outer2d<T>(f, c->outerF, false, st));
+        PSX_MARK(PSX_KIND_SPEC);
         PSX_HIP(launch_spec2d<T>(f, c->spec, true, s, st));
+        PSX_MARK(PSX_KIND_OUTER_INV);
         PSX_HIP(launch_outer2d<T>(f, c->outerI, true, st));
       } else {
+        PSX_MARK(PSX_KIND_SPEC);
         PSX_HIP(launch_spec2d<T>(f, c->spec, false, s, st));
       }
     } else {
+      PSX_MARK(PSX_KIND_LINE);
       PSX_HIP(launch_line3d<T>(f, c->line, 0, st));
+      PSX_MARK(PSX_KIND_MID_FWD);
       PSX_HIP(launch_ypass3d<T>(f, c->yF, false, f.rot ? 3 : 5, st));
+      PSX_MARK(PSX_KIND_SPEC);
       PSX_HIP(launch_spec3d<T>(f, c->spec, s, st));
+      PSX_MARK(PSX_KIND_MID_INV);
       PSX_HIP(launch_ypass3d<T>(f, c->yI, true, f.rot ? 6 : 3, st));
     }
   }
+  PSX_MARK(-1);
   return 0;
 }
 
@@ -405,6 +441,7 @@ int psx_create(const psx_flow_params* p, void* u0, void* acc, void* work, psx_ct
 
 int psx_destroy(psx_ctx* c) {
   if (!c) return 0;
+  for (hipEvent_t e : c->events) (void)hipEventDestroy(e);
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
   if (c->cap) (void)hipStreamDestroy(c->cap);
@@ -444,10 +481,13 @@ int psx_step_graph(psx_ctx* c, void* stream) {
   hipStream_t st = static_cast<hipStream_t>(stream);
   if (!c->gexec) {
     if (!c->cap) PSX_HIP(hipStreamCreateWithFlags(&c->cap, hipStreamNonBlocking));
+    const bool was_timing = c->timing;
+    c->timing = false;  // no event records inside the captured graph
     PSX_HIP(hipStreamBeginCapture(c->cap, hipStreamCaptureModeRelaxed));
     int r = c->fp64 ? enqueue_step<double>(c, c->cap) : enqueue_step<float>(c, c->cap);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->cap, &g);
+    c->timing = was_timing;
     if (r) {
       if (g) (void)hipGraphDestroy(g);
       return r;
@@ -457,6 +497,34 @@ int psx_step_graph(psx_ctx* c, void* stream) {
     PSX_HIP(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
   }
   PSX_HIP(hipGraphLaunch(c->gexec, st));
+  return 0;
+}
+
+int psx_timing_begin(psx_ctx* c) {
+  if (!c) return fail("psx_timing_begin: null context");
+  c->timing = true;
+  c->nrec = 0;
+  return 0;
+}
+
+int psx_timing_end(psx_ctx* c, void* stream, double* ms, int* count) {
+  if (!c || !ms || !count) return fail("psx_timing_end: null argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  c->timing = false;
+  PSX_HIP(hipStreamSynchronize(st));
+  for (int k = 0; k < PSX_NKINDS; ++k) {
+    ms[k] = 0.0;
+    count[k] = 0;
+  }
+  for (int i = 0; i + 1 < c->nrec; ++i) {
+    const int k = c->kinds[i];
+    if (k < 0 || k >= PSX_NKINDS) continue;
+    float t = 0.f;
+    PSX_HIP(hipEventElapsedTime(&t, c->events[i], c->events[i + 1]));
+    ms[k] += t;
+    count[k] += 1;
+  }
+  c->nrec = 0;
   return 0;
 }
 
