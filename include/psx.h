@@ -50,6 +50,8 @@ typedef struct psx_layout {
   int ldw;               /* padded retained half-axis width of a work row */
   int jm;                /* retained half-axis modes, 3 j < n */
   int four_step;         /* 1 if the full axes use the two-kernel split */
+  int e_line, e_spec, e_outer, e_mid; /* elements per thread of each kernel (0: unused) */
+  int t_line, t_spec, t_outer, t_mid; /* threads per work-group of each kernel */
 } psx_layout;
 
 /* Plan the kernels for p and report the buffer sizes.  Replaces the buffer and plan set-up

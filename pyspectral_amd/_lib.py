@@ -21,7 +21,9 @@ class FlowParams(ctypes.Structure):
 
 class Layout(ctypes.Structure):
     _fields_ = [("state_elems", c_longlong), ("work_elems", c_longlong), ("nfields", c_int),
-                ("ldw", c_int), ("jm", c_int), ("four_step", c_int)]
+                ("ldw", c_int), ("jm", c_int), ("four_step", c_int),
+                ("e_line", c_int), ("e_spec", c_int), ("e_outer", c_int), ("e_mid", c_int),
+                ("t_line", c_int), ("t_spec", c_int), ("t_outer", c_int), ("t_mid", c_int)]
 
 
 # name -> (restype, argtypes); the exported C ABI

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "psx_geo.h"
 #include "psx_kernels.h"
 
 using namespace psx;
@@ -143,6 +144,7 @@ struct psx_ctx {
   Flow<double> f64{};
   Launch line{}, spec{}, outerF{}, outerI{}, yF{}, yI{};
   void* tw = nullptr;
+  void* twp = nullptr;
   double* prm = nullptr;
   unsigned long long* stats = nullptr;
   double* partial = nullptr;
@@ -152,6 +154,7 @@ struct psx_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int nshell_max = 0;
+  bool ct = false;  // compile-time specialised 2D fp32 kernels for this size
   // per-launch event timing of eager steps (psx_timing_*)
   bool timing = false;
   std::vector<hipEvent_t> events;
@@ -197,41 +200,52 @@ int plan_flow(const psx_flow_params& p, psx_ctx* c, Flow<T>* f) {
   f->sstride = (long long)nrows * f->nh;
 
   const int nmax = p.dim == 2 ? 2 : 3;
-  if (!plan_rows(n, nrows, fp64, p.dim == 3, nmax, &c->line, &f->rowT, &f->prow))
-    return fail("psx: no line plan for n=%d", n);
   if (p.dim == 2) {
-    const size_t budget = 65536;
-    if ((size_t)n * T_ * csz <= budget || n <= 16) {
-      f->n1 = 1;
-      f->n2 = n;
-      if (!plan_cols(n, T_, 1, f->ldw, fp64, false, &c->spec, &f->pcol))
-        return fail("psx: no column plan for n=%d", n);
-    } else {
-      // n = n1 * n2, n2 >= n1, n2 the smallest divisor >= sqrt(n) that fits the budget
-      int best = 0;
-      for (int d = 2; d < n; ++d) {
-        if (n % d) continue;
-        if ((long long)d * d < n) continue;
-        if ((size_t)d * T_ * csz > budget) continue;
-        best = d;
-        break;
-      }
-      if (!best) return fail("psx: no four-step split for n=%d", n);
-      f->n2 = best;
-      f->n1 = n / best;
-      if (!plan_cols(f->n2, T_, f->n1, f->ldw, fp64, false, &c->spec, &f->pcol))
-        return fail("psx: no inner plan for n=%d", n);
-      if (!plan_cols(f->n1, T_, f->n2, f->ldw, fp64, false, &c->outerF, &f->pa))
-        return fail("psx: no outer plan for n=%d", n);
+    // one planner for the run-time and the compile-time specialised kernels
+    const Geo2D g = geo2d(n, fp64);
+    if (!g.ok) return fail("psx: no 2D plan for n=%d", n);
+    auto conv = [](const CtPlan& cp, FftPlan* fp) {
+      fp->L = cp.L;
+      fp->np = cp.np;
+      for (int i = 0; i < kMaxPasses; ++i) fp->rad[i] = i < cp.np ? cp.rad[i] : 1;
+    };
+    f->colT = f->yT = g.colT;
+    f->ldw = g.ldw;
+    f->fstride = (long long)nrows * f->ldw;
+    f->rowT = g.lineT;
+    f->n1 = g.n1;
+    f->n2 = g.n2;
+    conv(g.line, &f->prow);
+    conv(g.spec, &f->pcol);
+    if (g.n1 > 1) conv(g.outer, &f->pa);
+    c->line.E = g.lineE;
+    c->line.threads = g.lineNT;
+    c->line.blocks = n / g.lineT;
+    const size_t lds = (size_t)g.lineT * rows_pitch(n) * csz;
+    const size_t red = (size_t)nmax * g.lineNT * sizeof(double);
+    c->line.lds = lds > red ? lds : red;
+    const int njt = g.ldw / g.colT;
+    c->spec.E = g.specE;
+    c->spec.threads = g.specNT;
+    c->spec.blocks = g.n1 * njt;
+    c->spec.lds = (size_t)g.n2 * g.colT * csz;
+    if (g.n1 > 1) {
+      c->outerF.E = g.outE;
+      c->outerF.threads = g.outNT;
+      c->outerF.blocks = g.n2 * njt;
+      c->outerF.lds = (size_t)g.n1 * g.colT * csz;
       c->outerI = c->outerF;
     }
+    c->ct = !fp64 && ct2d_supported(n);
   } else {
+    if (!plan_rows(n, nrows, fp64, true, nmax, &c->line, &f->rowT, &f->prow))
+      return fail("psx: no line plan for n=%d", n);
     if ((size_t)n * T_ * csz > 98304) return fail("psx: 3D n=%d too large for one tile", n);
     f->n1 = 1;
     f->n2 = n;
     if (!plan_cols(n, T_, n, f->ldw, fp64, true, &c->spec, &f->pcol))
       return fail("psx: no x plan for n=%d", n);
-    if (!plan_cols(n, T_, n, f->ldw, fp64, false, &c->yF, &f->py))
+    if (!plan_cols(n, T_, n, f->ldw, fp64, true, &c->yF, &f->py))
       return fail("psx: no y plan for n=%d", n);
     c->yI = c->yF;
   }
@@ -241,6 +255,14 @@ int plan_flow(const psx_flow_params& p, psx_ctx* c, Flow<T>* f) {
   c->lay.ldw = f->ldw;
   c->lay.jm = f->jm;
   c->lay.four_step = f->n1 > 1;
+  c->lay.e_line = c->line.E;
+  c->lay.e_spec = c->spec.E;
+  c->lay.e_outer = f->n1 > 1 ? c->outerF.E : 0;
+  c->lay.e_mid = p.dim == 3 ? c->yF.E : 0;
+  c->lay.t_line = c->line.threads;
+  c->lay.t_spec = c->spec.threads;
+  c->lay.t_outer = f->n1 > 1 ? c->outerF.threads : 0;
+  c->lay.t_mid = p.dim == 3 ? c->yF.threads : 0;
   return 0;
 }
 
@@ -254,6 +276,43 @@ int make_twiddles(int n, void** out) {
   }
   PSX_HIP(hipMalloc(out, n * sizeof(cx<T>)));
   PSX_HIP(hipMemcpy(*out, h.data(), n * sizeof(cx<T>), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// Append the per-pass twiddle tables of plan p (forward and reversed radix order) to buf and
+// record their offsets in p->twoff.
+template <typename T>
+void append_pass_twiddles(FftPlan* p, std::vector<cx<T>>& buf) {
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (int order = 0; order < 2; ++order) {
+    int Ns = 1;
+    for (int q = 0; q < p->np; ++q) {
+      const int R = order == 0 ? p->rad[q] : p->rad[p->np - 1 - q];
+      p->twoff[order][q] = (int)buf.size();
+      if (Ns > 1) {
+        for (int m = 1; m < R; ++m)
+          for (int k = 0; k < Ns; ++k) {
+            const long double ang = two_pi * (long double)(m * k) / (long double)(Ns * R);
+            cx<T> w;
+            w.x = (T)(double)std::cos(ang);
+            w.y = (T)(double)(-std::sin(ang));
+            buf.push_back(w);
+          }
+      }
+      Ns *= R;
+    }
+  }
+}
+
+template <typename T>
+int make_pass_twiddles(Flow<T>* f, void** out) {
+  std::vector<cx<T>> buf;
+  FftPlan* plans[4] = {&f->prow, &f->pcol, &f->pa, &f->py};
+  for (FftPlan* p : plans)
+    if (p->np > 0) append_pass_twiddles<T>(p, buf);
+  if (buf.empty()) buf.push_back(cx<T>{(T)1, (T)0});
+  PSX_HIP(hipMalloc(out, buf.size() * sizeof(cx<T>)));
+  PSX_HIP(hipMemcpy(*out, buf.data(), buf.size() * sizeof(cx<T>), hipMemcpyHostToDevice));
   return 0;
 }
 
@@ -289,6 +348,13 @@ template <typename T>
 int enqueue_prepare(psx_ctx* c, hipStream_t st) {
   Flow<T>& f = flow<T>(c);
   if (f.dim == 2) {
+    if constexpr (sizeof(T) == 4) {
+      if (c->ct) {
+        PSX_HIP(launch_ct2d(f, 1, kStagePre, st));
+        if (f.n1 > 1) PSX_HIP(launch_ct2d(f, 3, 0, st));
+        return 0;
+      }
+    }
     if (f.n1 > 1) {
       PSX_HIP(launch_spec2d<T>(f, c->spec, true, kStagePre, st));
       PSX_HIP(launch_outer2d<T>(f, c->outerI, true, st));
@@ -306,7 +372,26 @@ template <typename T>
 int enqueue_step(psx_ctx* c, hipStream_t st) {
   Flow<T>& f = flow<T>(c);
   for (int s = 0; s < 4; ++s) {
-    if (f.dim == 2) {
+    bool done = false;
+    if constexpr (sizeof(T) == 4) {
+      if (c->ct) {
+        PSX_MARK(PSX_KIND_LINE);
+        PSX_HIP(launch_ct2d(f, 0, 0, st));
+        if (f.n1 > 1) {
+          PSX_MARK(PSX_KIND_OUTER_FWD);
+          PSX_HIP(launch_ct2d(f, 2, 0, st));
+        }
+        PSX_MARK(PSX_KIND_SPEC);
+        PSX_HIP(launch_ct2d(f, 1, s, st));
+        if (f.n1 > 1) {
+          PSX_MARK(PSX_KIND_OUTER_INV);
+          PSX_HIP(launch_ct2d(f, 3, 0, st));
+        }
+        done = true;
+      }
+    }
+    if (done) {
+    } else if (f.dim == 2) {
       PSX_MARK(PSX_KIND_LINE);
       PSX_HIP(launch_line2d<T>(f, c->line, 0, st));
       if (f.n1 > 1) {
@@ -339,7 +424,15 @@ template <typename T>
 int velocity_max(psx_ctx* c, double* out, hipStream_t st) {
   Flow<T>& f = flow<T>(c);
   PSX_HIP(launch_fill_u32(reinterpret_cast<unsigned*>(c->stats), 0u, 6, st));
-  if (f.dim == 2) {
+  bool done = false;
+  if constexpr (sizeof(T) == 4) {
+    if (c->ct) {
+      PSX_HIP(launch_ct2d(f, 0, kLineWantMax | kLineMaxOnly, st));
+      done = true;
+    }
+  }
+  if (done) {
+  } else if (f.dim == 2) {
     PSX_HIP(launch_line2d<T>(f, c->line, kLineWantMax | kLineMaxOnly, st));
   } else {
     PSX_HIP(launch_line3d<T>(f, c->line, kLineWantMax | kLineMaxOnly, st));
@@ -403,8 +496,11 @@ int psx_create(const psx_flow_params* p, void* u0, void* acc, void* work, psx_ct
   }
   const int n = p->n, nh = n / 2 + 1;
   r = c->fp64 ? make_twiddles<double>(n, &c->tw) : make_twiddles<float>(n, &c->tw);
+  if (!r)
+    r = c->fp64 ? make_pass_twiddles<double>(&c->f64, &c->twp)
+                : make_pass_twiddles<float>(&c->f32, &c->twp);
   if (r) {
-    delete c;
+    psx_destroy(c);
     return r;
   }
   const int nprm = 2 * nh + PrmLayout::kNScalars;
@@ -430,6 +526,7 @@ int psx_create(const psx_flow_params* p, void* u0, void* acc, void* work, psx_ct
     f.u0 = static_cast<cx<T>*>(u0);
     f.acc = static_cast<cx<T>*>(acc);
     f.tw = static_cast<const cx<T>*>(c->tw);
+    f.twp = static_cast<const cx<T>*>(c->twp);
     f.prm = c->prm;
     f.stats = c->stats;
   };
@@ -446,6 +543,7 @@ int psx_destroy(psx_ctx* c) {
   if (c->graph) (void)hipGraphDestroy(c->graph);
   if (c->cap) (void)hipStreamDestroy(c->cap);
   if (c->tw) (void)hipFree(c->tw);
+  if (c->twp) (void)hipFree(c->twp);
   if (c->prm) (void)hipFree(c->prm);
   if (c->stats) (void)hipFree(c->stats);
   if (c->partial) (void)hipFree(c->partial);

@@ -15,9 +15,9 @@
 // list, so the radix that ends a forward transform starts the inverse one: both see the
 // same butterfly-to-register assignment and can be chained in registers.
 //
-// Twiddles: one table tw[i] = exp(-2 pi i * i / N), i < N, generated in FP64 on the host and
-// rounded once to the working precision; every transform of length L | N indexes into it.
-// Constant twiddles inside the radix-8/16/32 butterflies are correctly rounded literals.
+// Twiddles: per-pass tables (see FftPlan::twoff) generated in FP64 on the host and rounded
+// once to the working precision; the inverse uses their conjugates.  Constant twiddles inside
+// the radix-8/16/32 butterflies are correctly rounded literals.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -205,6 +205,12 @@ struct FftPlan {
   int L;                 // transform length
   int np;                // number of passes
   int rad[kMaxPasses];   // radices, forward order
+  // Offsets into the context's pass-twiddle buffer: twoff[0][p] for pass p of the forward
+  // order, twoff[1][q] for pass q of the reversed order (inverse transforms).  Pass p with
+  // radix R after Ns = (product of earlier radices) owns (R-1)*Ns entries
+  // tw[off + (m-1)*Ns + k] = exp(-2 pi i m k / (Ns R)), m in [1, R), k in [0, Ns), so the
+  // lanes of a wave (consecutive k) read consecutive entries.
+  int twoff[2][kMaxPasses];
 };
 
 // Column tiles: sequence t is a column, element e a row; LDS address e*T + t.  Lanes map to
@@ -226,8 +232,13 @@ struct RowsLayout {
   int LP;  // padded row pitch
   PSX_HD int addr(int e, int t) const { return t * LP + e + (e >> 4); }
   PSX_HD void decomp(int g, int Lr, int& j, int& t) const {
-    t = g / Lr;
-    j = g - t * Lr;
+    if ((Lr & (Lr - 1)) == 0) {
+      t = g >> (__builtin_ctz(Lr));
+      j = g & (Lr - 1);
+    } else {
+      t = g / Lr;
+      j = g - t * Lr;
+    }
   }
 };
 
@@ -237,12 +248,13 @@ PSX_HD int rows_pitch(int L) { return L + (L >> 4) + 1; }
 // p > 0 it is re-read from LDS) and output q = m on exit.
 template <int E, int R, bool INV, typename T, class Lay>
 __device__ __forceinline__ void fft_pass(cx<T> (&v)[E], cx<T>* lds, bool from_lds,
-                                         bool to_lds, int L, int Ns, int twmul,
+                                         bool to_lds, int L, int Ns,
                                          const cx<T>* __restrict__ tw, const Lay& lay, int tid,
                                          int NT) {
   static_assert(E % R == 0, "radix must divide the per-thread element count");
   constexpr int NB = E / R;
   const int Lr = L / R;
+  const bool ns_pow2 = (Ns & (Ns - 1)) == 0;
   if (from_lds) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
@@ -257,11 +269,12 @@ __device__ __forceinline__ void fft_pass(cx<T> (&v)[E], cx<T>* lds, bool from_ld
     int j, t;
     lay.decomp(tid + b * NT, Lr, j, t);
     if (Ns > 1) {
-      const int k = j % Ns;
+      const int k = ns_pow2 ? (j & (Ns - 1)) : j % Ns;
+      const cx<T>* __restrict__ w = tw + k;
 #pragma unroll
       for (int m = 1; m < R; ++m) {
-        const cx<T> w = tw[m * k * twmul];
-        v[b * R + m] = INV ? cmulc(v[b * R + m], w) : cmul(v[b * R + m], w);
+        const cx<T> wm = w[(m - 1) * Ns];
+        v[b * R + m] = INV ? cmulc(v[b * R + m], wm) : cmul(v[b * R + m], wm);
       }
     }
     dft<R, INV>(&v[b * R]);
@@ -272,7 +285,7 @@ __device__ __forceinline__ void fft_pass(cx<T> (&v)[E], cx<T>* lds, bool from_ld
     for (int b = 0; b < NB; ++b) {
       int j, t;
       lay.decomp(tid + b * NT, Lr, j, t);
-      const int k = j % Ns;
+      const int k = ns_pow2 ? (j & (Ns - 1)) : j % Ns;
       const int base = (j - k) * R + k;
 #pragma unroll
       for (int q = 0; q < R; ++q) lds[lay.addr(base + q * Ns, t)] = v[b * R + q];
@@ -282,39 +295,32 @@ __device__ __forceinline__ void fft_pass(cx<T> (&v)[E], cx<T>* lds, bool from_ld
 }
 
 // Full transform.  On entry v holds the first pass's inputs in register order (see
-// reg_elem); on exit it holds the last pass's outputs in register order.
+// reg_elem); on exit it holds the last pass's outputs in register order.  twp is the
+// context's pass-twiddle buffer (FftPlan::twoff indexes into it).
 template <int E, bool INV, typename T, class Lay>
-__device__ __forceinline__ void fft_exec(cx<T> (&v)[E], cx<T>* lds, const FftPlan& pl, int twN,
-                                         const cx<T>* __restrict__ tw, const Lay& lay, int tid,
+__device__ __forceinline__ void fft_exec(cx<T> (&v)[E], cx<T>* lds, const FftPlan& pl,
+                                         const cx<T>* __restrict__ twp, const Lay& lay, int tid,
                                          int NT) {
   int Ns = 1;
   for (int p = 0; p < pl.np; ++p) {
     const int R = INV ? pl.rad[pl.np - 1 - p] : pl.rad[p];
-    const int twmul = twN / (Ns * R);
+    const cx<T>* tw = twp + pl.twoff[INV ? 1 : 0][p];
     const bool from_lds = p > 0;
     const bool to_lds = p < pl.np - 1;
     switch (R) {
-      case 2:
-        if constexpr (E % 2 == 0) fft_pass<E, 2, INV>(v, lds, from_lds, to_lds, pl.L, Ns, twmul, tw, lay, tid, NT);
-        break;
-      case 3:
-        if constexpr (E % 3 == 0) fft_pass<E, 3, INV>(v, lds, from_lds, to_lds, pl.L, Ns, twmul, tw, lay, tid, NT);
-        break;
-      case 4:
-        if constexpr (E % 4 == 0) fft_pass<E, 4, INV>(v, lds, from_lds, to_lds, pl.L, Ns, twmul, tw, lay, tid, NT);
-        break;
-      case 5:
-        if constexpr (E % 5 == 0) fft_pass<E, 5, INV>(v, lds, from_lds, to_lds, pl.L, Ns, twmul, tw, lay, tid, NT);
-        break;
-      case 8:
-        if constexpr (E % 8 == 0) fft_pass<E, 8, INV>(v, lds, from_lds, to_lds, pl.L, Ns, twmul, tw, lay, tid, NT);
-        break;
-      case 16:
-        if constexpr (E % 16 == 0) fft_pass<E, 16, INV>(v, lds, from_lds, to_lds, pl.L, Ns, twmul, tw, lay, tid, NT);
-        break;
-      case 32:
-        if constexpr (E % 32 == 0) fft_pass<E, 32, INV>(v, lds, from_lds, to_lds, pl.L, Ns, twmul, tw, lay, tid, NT);
-        break;
+#define PSX_RADIX(RV)                                                                        \
+  case RV:                                                                                  \
+    if constexpr (E % RV == 0)                                                              \
+      fft_pass<E, RV, INV>(v, lds, from_lds, to_lds, pl.L, Ns, tw, lay, tid, NT);           \
+    break;
+      PSX_RADIX(2)
+      PSX_RADIX(3)
+      PSX_RADIX(4)
+      PSX_RADIX(5)
+      PSX_RADIX(8)
+      PSX_RADIX(16)
+      PSX_RADIX(32)
+#undef PSX_RADIX
       default:
         break;
     }

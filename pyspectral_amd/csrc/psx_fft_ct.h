@@ -1,0 +1,106 @@
+// Compile-time planned variant of the FFT engine in psx_fft.h.
+//
+// Same pass formulation, register order and twiddle-table layout, but the plan (length,
+// radices), the work-group size and the tile shape are template parameters, so every index
+// in a pass folds to shifts, masks and constants; only the thread index is a run-time value.
+// The kernels built on it are specialised per grid size (psx_geo.h, PSX_CT2D_SIZES).
+#pragma once
+
+#include "psx_fft.h"
+#include "psx_geo.h"
+
+namespace psx {
+
+// Row tiles: T_ sequences of length L_ stored contiguously in LDS, one pad element per 16.
+template <int L_, int T_>
+struct RowsCT {
+  static constexpr int L = L_, T = T_, LP = ct_rows_pitch(L_);
+  static PSX_HD int addr(int e, int t) { return t * LP + e + (e >> 4); }
+  template <int Lr>
+  static PSX_HD void decomp(int g, int& j, int& t) {
+    t = g / Lr;
+    j = g - t * Lr;
+  }
+};
+
+// Column tiles: T_ columns (power of two) side by side; lanes walk the columns.
+template <int T_>
+struct ColsCT {
+  static constexpr int T = T_;
+  static PSX_HD int addr(int e, int t) { return e * T + t; }
+  template <int Lr>
+  static PSX_HD void decomp(int g, int& j, int& t) {
+    t = g % T;
+    j = g / T;
+  }
+};
+
+template <int E, int R, bool INV, int L, int Ns, int NT, class Lay, typename T>
+__device__ __forceinline__ void ct_pass(cx<T> (&v)[E], cx<T>* lds, bool from_lds, bool to_lds,
+                                        const cx<T>* __restrict__ tw, int tid) {
+  static_assert(E % R == 0, "radix must divide the per-thread element count");
+  constexpr int NB = E / R;
+  constexpr int Lr = L / R;
+  if (from_lds) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      int j, t;
+      Lay::template decomp<Lr>(tid + b * NT, j, t);
+#pragma unroll
+      for (int m = 0; m < R; ++m) v[b * R + m] = lds[Lay::addr(j + m * Lr, t)];
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    int j, t;
+    Lay::template decomp<Lr>(tid + b * NT, j, t);
+    if constexpr (Ns > 1) {
+      const int k = j % Ns;
+#pragma unroll
+      for (int m = 1; m < R; ++m) {
+        const cx<T> w = tw[(m - 1) * Ns + k];
+        v[b * R + m] = INV ? cmulc(v[b * R + m], w) : cmul(v[b * R + m], w);
+      }
+    }
+    dft<R, INV>(&v[b * R]);
+  }
+  if (to_lds) {
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      int j, t;
+      Lay::template decomp<Lr>(tid + b * NT, j, t);
+      const int k = j % Ns;
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int q = 0; q < R; ++q) lds[Lay::addr(base + q * Ns, t)] = v[b * R + q];
+    }
+    __syncthreads();
+  }
+}
+
+// Whole transform; tw_order points at the start of this plan's forward (INV = false) or
+// reversed-order (INV = true) twiddle table.
+template <int E, bool INV, int NT, class Lay, CtPlan P, typename T>
+__device__ __forceinline__ void fft_ct(cx<T> (&v)[E], cx<T>* lds,
+                                       const cx<T>* __restrict__ tw_order, int tid) {
+  static_for<0, P.np>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int R = ct_radix(P, INV, q);
+    constexpr int Ns = ct_ns(P, INV, q);
+    constexpr int off = ct_twoff(P, INV, q);
+    ct_pass<E, R, INV, P.L, Ns, NT, Lay>(v, lds, q > 0, q < P.np - 1, tw_order + off, tid);
+  });
+}
+
+// Register slot i of a pass of radix R holds element e of sequence t.
+template <int R, int L, int NT, class Lay>
+PSX_HD void ct_elem(int i, int tid, int& e, int& t) {
+  const int b = i / R;
+  const int m = i - b * R;
+  int j;
+  Lay::template decomp<L / R>(tid + b * NT, j, t);
+  e = j + m * (L / R);
+}
+
+}  // namespace psx

@@ -28,7 +28,8 @@ struct Flow {
   cx<T>* W;          // work fields
   cx<T>* u0;         // state
   cx<T>* acc;        // RK accumulator
-  const cx<T>* tw;   // twiddles, length n
+  const cx<T>* tw;   // twiddles exp(-2 pi i m / n), m < n (four-step and diagnostics)
+  const cx<T>* twp;  // per-pass twiddle tables of all plans (FftPlan::twoff)
   const double* prm; // 2*nh table entries followed by PrmLayout scalars
   unsigned long long* stats;  // CFL maxima max|u_i| as FP64 bit patterns, 3 entries
   long long fstride; // complex elements per work field (nrows * ldw)

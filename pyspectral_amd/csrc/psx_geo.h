@@ -1,0 +1,147 @@
+// Compile-time and run-time planning shared by the host planner and the kernels.
+//
+// One constexpr planner decides, for a transform length and an elements-per-thread count,
+// the radix list; and, for a 2D grid size, every launch shape (work-group size, tile width,
+// four-step split).  The host fills the run-time Flow/Launch structures from it and the
+// specialised kernels evaluate the same functions at compile time, so both always agree on
+// register order and on the layout of the pass-twiddle tables.
+#pragma once
+
+namespace psx {
+
+constexpr int kMaxRad = 8;
+
+struct CtPlan {
+  int L = 0;
+  int np = 0;
+  int rad[kMaxRad] = {1, 1, 1, 1, 1, 1, 1, 1};
+};
+
+// Radix list for length L with every radix dividing E; balanced power-of-two radices
+// (<= 32) first, then 3s and 5s.  np == 0 if impossible.
+constexpr CtPlan ct_plan(int L, int E) {
+  CtPlan p{};
+  int l = L, a = 0, b3 = 0, b5 = 0;
+  if (L <= 0 || E <= 0) return p;
+  while (l % 2 == 0) { l /= 2; ++a; }
+  while (l % 3 == 0) { l /= 3; ++b3; }
+  while (l % 5 == 0) { l /= 5; ++b5; }
+  if (l != 1) return p;
+  if (b3 && E % 3) return p;
+  if (b5 && E % 5) return p;
+  int lmax = 0;
+  while (lmax < 5 && E % (1 << (lmax + 1)) == 0) ++lmax;
+  if (a > 0 && lmax == 0) return p;
+  const int np2 = a == 0 ? 0 : (a + lmax - 1) / lmax;
+  const int np = np2 + b3 + b5;
+  if (np > kMaxRad || np == 0) return p;
+  p.L = L;
+  p.np = np;
+  int k = 0;
+  for (int i = 0; i < np2; ++i) {
+    const int bits = a / np2 + (i < a % np2 ? 1 : 0);
+    p.rad[k++] = 1 << bits;
+  }
+  for (int i = 0; i < b3; ++i) p.rad[k++] = 3;
+  for (int i = 0; i < b5; ++i) p.rad[k++] = 5;
+  return p;
+}
+
+// Radix of pass q in the forward (inv = false) or reversed (inv = true) order.
+constexpr int ct_radix(const CtPlan& p, bool inv, int q) {
+  return inv ? p.rad[p.np - 1 - q] : p.rad[q];
+}
+// Product of the radices before pass q.
+constexpr int ct_ns(const CtPlan& p, bool inv, int q) {
+  int ns = 1;
+  for (int i = 0; i < q; ++i) ns *= ct_radix(p, inv, i);
+  return ns;
+}
+// Offset of pass q's twiddles inside one order's table (pass 0 owns none).
+constexpr int ct_twoff(const CtPlan& p, bool inv, int q) {
+  int off = 0;
+  for (int i = 1; i < q; ++i) off += (ct_radix(p, inv, i) - 1) * ct_ns(p, inv, i);
+  return off;
+}
+// Entries of one order's table.
+constexpr int ct_twsize(const CtPlan& p, bool inv) { return ct_twoff(p, inv, p.np); }
+
+constexpr int ct_rows_pitch(int L) { return L + (L >> 4) + 1; }
+constexpr int ct_retained(int n) { return (n - 1) / 3 + 1; }
+
+// Launch geometry of the 2D kernels for grid size n in fp32 (csz = 8) or fp64 (csz = 16).
+struct Geo2D {
+  bool ok = false;
+  int n = 0, jm = 0, colT = 0, ldw = 0, n1 = 1, n2 = 0;
+  int lineE = 0, lineNT = 0, lineT = 0;
+  int specE = 0, specNT = 0;
+  int outE = 0, outNT = 0;
+  CtPlan line{}, spec{}, outer{};
+};
+
+constexpr int ct_pick_E(int L, bool fp64) {
+  if (L % 3 == 0) return fp64 ? 12 : 24;
+  return 16;
+}
+
+constexpr Geo2D geo2d(int n, bool fp64) {
+  Geo2D g{};
+  const int csz = fp64 ? 16 : 8;
+  g.n = n;
+  g.jm = ct_retained(n);
+  int T = fp64 ? 8 : 16;
+  while (T > 1 && T / 2 >= g.jm) T /= 2;
+  g.colT = T;
+  g.ldw = (g.jm + T - 1) / T * T;
+  // line kernel: whole rows
+  {
+    int E = ct_pick_E(n, fp64);
+    int t = 1;
+    while (n * t / E < 128 && t * 2 <= n) t *= 2;
+    int nt = n * t / E;
+    if (nt > 512 && !fp64 && n % 3 != 0) {
+      E = 32;
+      nt = n * t / E;
+    }
+    g.line = ct_plan(n, E);
+    if (g.line.np == 0 || nt > 512 || (n * t) % E) return g;
+    g.lineE = E;
+    g.lineNT = nt;
+    g.lineT = t;
+  }
+  // column kernels: one pass if a whole-column tile fits 64 KB, else the four-step split
+  const int budget = 65536;
+  auto cols = [&](int L, int& E, int& nt, CtPlan& pl) {
+    E = ct_pick_E(L, fp64);
+    while (E > 4 && (L * T) % E) E /= 2;
+    pl = ct_plan(L, E);
+    nt = L * T / E;
+    return pl.np > 0 && nt >= 1 && nt <= 512 && (L * T) % E == 0;
+  };
+  if (n * T * csz <= budget || n <= 16) {
+    g.n1 = 1;
+    g.n2 = n;
+    if (!cols(n, g.specE, g.specNT, g.spec)) return g;
+  } else {
+    int best = 0;
+    for (int d = 2; d < n; ++d) {
+      if (n % d) continue;
+      if (d * d < n) continue;
+      if (d * T * csz > budget) continue;
+      best = d;
+      break;
+    }
+    if (!best) return g;
+    g.n2 = best;
+    g.n1 = n / best;
+    if (!cols(g.n2, g.specE, g.specNT, g.spec)) return g;
+    if (!cols(g.n1, g.outE, g.outNT, g.outer)) return g;
+  }
+  g.ok = true;
+  return g;
+}
+
+// Grid sizes with compile-time specialised 2D fp32 kernels.
+#define PSX_CT2D_SIZES(X) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
+
+}  // namespace psx

@@ -19,7 +19,7 @@ namespace psx {
 
 // ------------------------------------------------------------------------------ line kernel
 template <typename T, int E>
-__global__ void __launch_bounds__(512) k_line2d(Flow<T> f, int flags) {
+__global__ void __launch_bounds__(512, 1) k_line2d(Flow<T> f, int flags) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int tid = threadIdx.x, NT = blockDim.x;
@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(512) k_line2d(Flow<T> f, int flags) {
       v[i] = z;
     }
   }
-  fft_exec<E, true>(v, lds, f.prow, f.n, f.tw, lay, tid, NT);
+  fft_exec<E, true>(v, lds, f.prow, f.twp, lay, tid, NT);
 
   if (flags & kLineWantMax) {
     double m[2] = {0.0, 0.0};
@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(512) k_line2d(Flow<T> f, int flags) {
     const T u = v[i].x, w = v[i].y;
     v[i] = mkc((w - u) * (w + u), u * w);
   }
-  fft_exec<E, false>(v, lds, f.prow, f.n, f.tw, lay, tid, NT);
+  fft_exec<E, false>(v, lds, f.prow, f.twp, lay, tid, NT);
 
   __syncthreads();
   {
@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(512) k_line2d(Flow<T> f, int flags) {
 // of the four-step split n = n1 n2; the group of rows g*n2 .. g*n2 + n2-1 holds, for the
 // outer index k1 = g, the modes k = k1 + n1 k2 after the forward transform.
 template <typename T, int E, bool FOUR>
-__global__ void __launch_bounds__(512) k_spec2d(Flow<T> f, int stage) {
+__global__ void __launch_bounds__(512, 1) k_spec2d(Flow<T> f, int stage) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int tid = threadIdx.x, NT = blockDim.x;
@@ -136,8 +136,8 @@ __global__ void __launch_bounds__(512) k_spec2d(Flow<T> f, int stage) {
       va[i] = W0[a];
       vb[i] = W1[a];
     }
-    fft_exec<E, false>(va, lds, f.pcol, n, f.tw, lay, tid, NT);
-    fft_exec<E, false>(vb, lds, f.pcol, n, f.tw, lay, tid, NT);
+    fft_exec<E, false>(va, lds, f.pcol, f.twp, lay, tid, NT);
+    fft_exec<E, false>(vb, lds, f.pcol, f.twp, lay, tid, NT);
   }
   {
     const int R = last_radix(f.pcol, false);
@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(512) k_spec2d(Flow<T> f, int stage) {
       vb[i] = ov;
     }
   }
-  fft_exec<E, true>(va, lds, f.pcol, n, f.tw, lay, tid, NT);
+  fft_exec<E, true>(va, lds, f.pcol, f.twp, lay, tid, NT);
   {
     const int R = last_radix(f.pcol, true);
 #pragma unroll
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(512) k_spec2d(Flow<T> f, int stage) {
       W0[(long long)(eo + e) * f.ldw + j0 + t] = z;
     }
   }
-  fft_exec<E, true>(vb, lds, f.pcol, n, f.tw, lay, tid, NT);
+  fft_exec<E, true>(vb, lds, f.pcol, f.twp, lay, tid, NT);
   {
     const int R = last_radix(f.pcol, true);
 #pragma unroll
@@ -205,7 +205,7 @@ __global__ void __launch_bounds__(512) k_spec2d(Flow<T> f, int stage) {
 // w_n^{x2 k1}, store at row k1 n2 + x2.  Inverse: transform over k1 -> x1 (the conjugate
 // twiddle was applied by the inner kernel), store at row n2 x1 + x2.  Both work fields.
 template <typename T, int E, bool INV>
-__global__ void __launch_bounds__(512) k_outer2d(Flow<T> f) {
+__global__ void __launch_bounds__(512, 1) k_outer2d(Flow<T> f) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int tid = threadIdx.x, NT = blockDim.x;
@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(512) k_outer2d(Flow<T> f) {
         v[i] = Wf[(long long)(e * f.n2 + x2) * f.ldw + j0 + t];
       }
     }
-    fft_exec<E, INV>(v, lds, f.pa, f.n, f.tw, lay, tid, NT);
+    fft_exec<E, INV>(v, lds, f.pa, f.twp, lay, tid, NT);
     {
       const int R = last_radix(f.pa, INV);
 #pragma unroll
@@ -246,12 +246,12 @@ template <typename T>
 hipError_t launch_line2d(const Flow<T>& f, const Launch& lc, int flags, hipStream_t st) {
   switch (lc.E) {
 #define PSX_CASE(EV)                                                                      \
-  case EV: {                                                                              \
+  case EV: if constexpr (e_ok_2d<T>(EV)) {                                                                              \
     auto fn = k_line2d<T, EV>;                                                            \
     static const bool attr_ = set_lds_max((const void*)fn); (void)attr_;                                         \
     hipLaunchKernelGGL(fn, dim3(lc.blocks), dim3(lc.threads), lc.lds, st, f, flags);      \
     break;                                                                                \
-  }
+  } else { return hipErrorInvalidValue; }
     PSX_E_CASES
 #undef PSX_CASE
     default:
@@ -265,7 +265,7 @@ hipError_t launch_spec2d(const Flow<T>& f, const Launch& lc, bool four, int stag
                          hipStream_t st) {
   switch (lc.E) {
 #define PSX_CASE(EV)                                                                      \
-  case EV: {                                                                              \
+  case EV: if constexpr (e_ok_2d<T>(EV)) {                                                                              \
     if (four) {                                                                           \
       auto fn = k_spec2d<T, EV, true>;                                                    \
       static const bool attr_ = set_lds_max((const void*)fn); (void)attr_;                                       \
@@ -276,7 +276,7 @@ hipError_t launch_spec2d(const Flow<T>& f, const Launch& lc, bool four, int stag
       hipLaunchKernelGGL(fn, dim3(lc.blocks), dim3(lc.threads), lc.lds, st, f, stage);    \
     }                                                                                     \
     break;                                                                                \
-  }
+  } else { return hipErrorInvalidValue; }
     PSX_E_CASES
 #undef PSX_CASE
     default:
@@ -289,7 +289,7 @@ template <typename T>
 hipError_t launch_outer2d(const Flow<T>& f, const Launch& lc, bool inv, hipStream_t st) {
   switch (lc.E) {
 #define PSX_CASE(EV)                                                                      \
-  case EV: {                                                                              \
+  case EV: if constexpr (e_ok_2d<T>(EV)) {                                                                              \
     if (inv) {                                                                            \
       auto fn = k_outer2d<T, EV, true>;                                                   \
       static const bool attr_ = set_lds_max((const void*)fn); (void)attr_;                                       \
@@ -300,7 +300,7 @@ hipError_t launch_outer2d(const Flow<T>& f, const Launch& lc, bool inv, hipStrea
       hipLaunchKernelGGL(fn, dim3(lc.blocks), dim3(lc.threads), lc.lds, st, f);           \
     }                                                                                     \
     break;                                                                                \
-  }
+  } else { return hipErrorInvalidValue; }
     PSX_E_CASES
 #undef PSX_CASE
     default:

@@ -1,0 +1,308 @@
+// 2D fp32 stage kernels specialised per grid size (PSX_CT2D_SIZES): the same algorithms as
+// psx_k2d.hip with the plans, launch shapes and HBM strides fixed at compile time by the
+// shared planner geo2d() (psx_geo.h).
+#include "psx_fft_ct.h"
+#include "psx_kernels.h"
+
+namespace psx {
+
+template <int N>
+struct G2 {
+  static constexpr Geo2D g = geo2d(N, false);
+  static_assert(g.ok, "no 2D plan for this size");
+};
+
+// --------------------------------------------------------------------------- line kernel
+template <int N>
+__global__ void __launch_bounds__(G2<N>::g.lineNT, 1) kc_line2d(Flow<float> f, int flags) {
+  using T = float;
+  constexpr Geo2D G = G2<N>::g;
+  constexpr int E = G.lineE, NT = G.lineNT, RT = G.lineT, L = N, JM = G.jm, LDW = G.ldw;
+  constexpr long long FS = (long long)N * LDW;
+  using Lay = RowsCT<L, RT>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int row0 = blockIdx.x * RT;
+  cx<T>* U = f.W;
+  cx<T>* V = f.W + FS;
+  const cx<T>* tw_fwd = f.twp + f.prow.twoff[0][0];
+  const cx<T>* tw_rev = f.twp + f.prow.twoff[1][0];
+
+  cx<T> v[E];
+  {
+    constexpr int R = ct_radix(G.line, true, 0);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int e, t;
+      ct_elem<R, L, NT, Lay>(i, tid, e, t);
+      const long long base = (long long)(row0 + t) * LDW;
+      cx<T> z = czero<T>();
+      if (e == 0) {
+        z = mkc(U[base].x, V[base].x);
+      } else if (e < JM) {
+        const cx<T> a = U[base + e], b = V[base + e];
+        z = mkc(a.x - b.y, a.y + b.x);
+      } else if (e > L - JM) {
+        const cx<T> a = U[base + (L - e)], b = V[base + (L - e)];
+        z = mkc(a.x + b.y, b.x - a.y);
+      }
+      v[i] = z;
+    }
+  }
+  fft_ct<E, true, NT, Lay, G.line>(v, lds, tw_rev, tid);
+
+  if (flags & kLineWantMax) {
+    double m[2] = {0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      m[0] = fmax(m[0], abs_or_inf((double)v[i].x));
+      m[1] = fmax(m[1], abs_or_inf((double)v[i].y));
+    }
+    block_max_to<2>(m, f.stats, reinterpret_cast<double*>(smem_raw));
+  }
+  if (flags & kLineMaxOnly) return;
+
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const T u = v[i].x, w = v[i].y;
+    v[i] = mkc((w - u) * (w + u), u * w);
+  }
+  fft_ct<E, false, NT, Lay, G.line>(v, lds, tw_fwd, tid);
+
+  __syncthreads();
+  {
+    constexpr int R = ct_radix(G.line, false, G.line.np - 1);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int e, t;
+      ct_elem<R, L, NT, Lay>(i, tid, e, t);
+      lds[Lay::addr(e, t)] = v[i];
+    }
+  }
+  __syncthreads();
+  const T h = (T)0.5;
+  // split the transform of P + iQ into the half spectra of P and Q, columns k < JM
+#pragma unroll
+  for (int t = 0; t < RT; ++t) {
+    const long long rb = (long long)(row0 + t) * LDW;
+    for (int k = tid; k < JM; k += NT) {
+      const cx<T> z = lds[Lay::addr(k, t)];
+      const cx<T> zc = conjg(lds[Lay::addr(k == 0 ? 0 : L - k, t)]);
+      U[rb + k] = cscl(h, cadd(z, zc));
+      V[rb + k] = cscl(h, mul_mi(csub(z, zc)));
+    }
+  }
+}
+
+// ----------------------------------------------------------------------- spectral kernel
+template <int N>
+__global__ void __launch_bounds__(G2<N>::g.specNT, 1) kc_spec2d(Flow<float> f, int stage) {
+  using T = float;
+  constexpr Geo2D G = G2<N>::g;
+  constexpr bool FOUR = G.n1 > 1;
+  constexpr int E = G.specE, NT = G.specNT, CT = G.colT, L = G.n2, N1 = G.n1, JM = G.jm;
+  constexpr int LDW = G.ldw, NH = N / 2 + 1;
+  constexpr long long FS = (long long)N * LDW;
+  constexpr int NJT = LDW / CT;
+  using Lay = ColsCT<CT>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int grp = blockIdx.x / NJT;
+  const int j0 = (blockIdx.x - grp * NJT) * CT;
+  const int eo = FOUR ? grp * L : 0;
+  cx<T>* W0 = f.W;
+  cx<T>* W1 = f.W + FS;
+  const cx<T>* tw_fwd = f.twp + f.pcol.twoff[0][0];
+  const cx<T>* tw_rev = f.twp + f.pcol.twoff[1][0];
+
+  cx<T> va[E], vb[E];
+  if (stage != kStagePre) {
+    constexpr int R = ct_radix(G.spec, false, 0);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int e, t;
+      ct_elem<R, L, NT, Lay>(i, tid, e, t);
+      const long long a = (long long)(eo + e) * LDW + j0 + t;
+      va[i] = W0[a];
+      vb[i] = W1[a];
+    }
+    fft_ct<E, false, NT, Lay, G.spec>(va, lds, tw_fwd, tid);
+    fft_ct<E, false, NT, Lay, G.spec>(vb, lds, tw_fwd, tid);
+  }
+  {
+    constexpr int R = ct_radix(G.spec, false, G.spec.np - 1);
+    const IfScalars s = load_if_scalars(f.prm, NH);
+    const T dt = (T)f.prm[2 * NH + PrmLayout::kDt];
+    const T scale = f.scale;
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int e, t;
+      ct_elem<R, L, NT, Lay>(i, tid, e, t);
+      const int k = FOUR ? grp + N1 * e : e;
+      const int kx = k < N / 2 ? k : k - N;
+      const int ky = j0 + t;
+      const int ax = kx < 0 ? -kx : kx;
+      cx<T> ou = czero<T>(), ov = czero<T>();
+      if (3 * ax < N && ky < JM) {
+        const long long si = (long long)k * NH + ky;
+        const int k2 = kx * kx + ky * ky;
+        cx<T> nxt;
+        if (stage == kStagePre) {
+          nxt = f.u0[si];
+        } else {
+          const T fx = (T)kx, fy = (T)ky;
+          const cx<T> Nl = cadd(cscl(fx * fy, va[i]), cscl(fx * fx - fy * fy, vb[i]));
+          double E1, Eh;
+          if_factors(f.prm, NH, s, ax, ky, -1, k2, E1, Eh);
+          nxt = rk_update<T>(stage, dt, (T)E1, (T)Eh, Nl, f.u0 + si, f.acc + si);
+        }
+        const T ik2 = k2 > 0 ? scale / (T)k2 : (T)0;
+        const cx<T> psi = cscl(ik2, nxt);
+        ou = cscl((T)ky, mul_i(psi));
+        ov = cscl((T)kx, mul_mi(psi));
+      }
+      va[i] = ou;
+      vb[i] = ov;
+    }
+  }
+  constexpr int RS = ct_radix(G.spec, true, G.spec.np - 1);
+  fft_ct<E, true, NT, Lay, G.spec>(va, lds, tw_rev, tid);
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    int e, t;
+    ct_elem<RS, L, NT, Lay>(i, tid, e, t);
+    cx<T> z = va[i];
+    if (FOUR) z = cmulc(z, f.tw[e * grp]);
+    W0[(long long)(eo + e) * LDW + j0 + t] = z;
+  }
+  fft_ct<E, true, NT, Lay, G.spec>(vb, lds, tw_rev, tid);
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    int e, t;
+    ct_elem<RS, L, NT, Lay>(i, tid, e, t);
+    cx<T> z = vb[i];
+    if (FOUR) z = cmulc(z, f.tw[e * grp]);
+    W1[(long long)(eo + e) * LDW + j0 + t] = z;
+  }
+}
+
+// ------------------------------------------------------------------- outer four-step pass
+template <int N, bool INV>
+__global__ void __launch_bounds__(G2<N>::g.outNT > 0 ? G2<N>::g.outNT : 64, 1)
+    kc_outer2d(Flow<float> f) {
+  using T = float;
+  constexpr Geo2D G = G2<N>::g;
+  if constexpr (G.n1 > 1) {
+    constexpr int E = G.outE, NT = G.outNT, CT = G.colT, L = G.n1, N2 = G.n2;
+    constexpr int LDW = G.ldw, NJT = LDW / CT;
+    constexpr long long FS = (long long)N * LDW;
+    using Lay = ColsCT<CT>;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+    const int tid = threadIdx.x;
+    const int x2 = blockIdx.x / NJT;
+    const int j0 = (blockIdx.x - x2 * NJT) * CT;
+    const cx<T>* tw_o = f.twp + f.pa.twoff[INV ? 1 : 0][0];
+    constexpr int R0 = ct_radix(G.outer, INV, 0);
+    constexpr int RL = ct_radix(G.outer, INV, G.outer.np - 1);
+    cx<T> v0[E], v1[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int e, t;
+      ct_elem<R0, L, NT, Lay>(i, tid, e, t);
+      const long long a = (long long)(e * N2 + x2) * LDW + j0 + t;
+      v0[i] = f.W[a];
+      v1[i] = f.W[FS + a];
+    }
+    fft_ct<E, INV, NT, Lay, G.outer>(v0, lds, tw_o, tid);
+    fft_ct<E, INV, NT, Lay, G.outer>(v1, lds, tw_o, tid);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int e, t;
+      ct_elem<RL, L, NT, Lay>(i, tid, e, t);
+      cx<T> z0 = v0[i], z1 = v1[i];
+      if (!INV) {
+        const cx<T> w = f.tw[e * x2];
+        z0 = cmul(z0, w);
+        z1 = cmul(z1, w);
+      }
+      const long long a = (long long)(e * N2 + x2) * LDW + j0 + t;
+      f.W[a] = z0;
+      f.W[FS + a] = z1;
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------ launchers
+bool ct2d_supported(int n) {
+  switch (n) {
+#define PSX_X(NV) case NV:
+    PSX_CT2D_SIZES(PSX_X)
+#undef PSX_X
+    return true;
+    default:
+      return false;
+  }
+}
+
+template <int N>
+static hipError_t ct_launch(const Flow<float>& f, int which, int arg, hipStream_t st) {
+  constexpr Geo2D G = G2<N>::g;
+  constexpr int NJT = G.ldw / G.colT;
+  switch (which) {
+    case 0: {
+      auto fn = kc_line2d<N>;
+      static const bool attr_ = set_lds_max((const void*)fn);
+      (void)attr_;
+      size_t lds = (size_t)G.lineT * ct_rows_pitch(N) * 8;
+      const size_t red = (size_t)2 * G.lineNT * sizeof(double);
+      if (lds < red) lds = red;
+      hipLaunchKernelGGL(fn, dim3(N / G.lineT), dim3(G.lineNT), lds, st, f, arg);
+      break;
+    }
+    case 1: {
+      auto fn = kc_spec2d<N>;
+      static const bool attr_ = set_lds_max((const void*)fn);
+      (void)attr_;
+      const size_t lds = (size_t)G.n2 * G.colT * 8;
+      hipLaunchKernelGGL(fn, dim3(G.n1 * NJT), dim3(G.specNT), lds, st, f, arg);
+      break;
+    }
+    case 2:
+    case 3: {
+      if constexpr (G.n1 > 1) {
+        const size_t lds = (size_t)G.n1 * G.colT * 8;
+        if (which == 2) {
+          auto fn = kc_outer2d<N, false>;
+          hipLaunchKernelGGL(fn, dim3(G.n2 * NJT), dim3(G.outNT), lds, st, f);
+        } else {
+          auto fn = kc_outer2d<N, true>;
+          hipLaunchKernelGGL(fn, dim3(G.n2 * NJT), dim3(G.outNT), lds, st, f);
+        }
+      } else {
+        return hipErrorInvalidValue;
+      }
+      break;
+    }
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// which: 0 line (arg = flags), 1 spectral (arg = stage), 2 outer forward, 3 outer inverse.
+hipError_t launch_ct2d(const Flow<float>& f, int which, int arg, hipStream_t st) {
+  switch (f.n) {
+#define PSX_X(NV) \
+  case NV:        \
+    return ct_launch<NV>(f, which, arg, st);
+    PSX_CT2D_SIZES(PSX_X)
+#undef PSX_X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace psx

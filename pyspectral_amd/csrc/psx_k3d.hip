@@ -107,7 +107,7 @@ __device__ __forceinline__ void line_store_real(const cx<T> (&v)[E], const Flow<
 }
 
 template <typename T, int E>
-__global__ void __launch_bounds__(512) k_line3d(Flow<T> f, int flags) {
+__global__ void __launch_bounds__(512, 1) k_line3d(Flow<T> f, int flags) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int tid = threadIdx.x, NT = blockDim.x;
@@ -120,9 +120,9 @@ __global__ void __launch_bounds__(512) k_line3d(Flow<T> f, int flags) {
   if (!f.rot) {
     cx<T> z1[E], z2[E];
     line_load_pair<T, E>(z1, f, 0, 1, row0, nrows, tid, NT, lay);
-    fft_exec<E, true>(z1, lds, f.prow, f.n, f.tw, lay, tid, NT);
+    fft_exec<E, true>(z1, lds, f.prow, f.twp, lay, tid, NT);
     line_load_pair<T, E>(z2, f, 2, -1, row0, nrows, tid, NT, lay);
-    fft_exec<E, true>(z2, lds, f.prow, f.n, f.tw, lay, tid, NT);
+    fft_exec<E, true>(z2, lds, f.prow, f.twp, lay, tid, NT);
     if (want_max) {
       double m[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -148,20 +148,20 @@ __global__ void __launch_bounds__(512) k_line3d(Flow<T> f, int flags) {
       z2[i] = mkc(u * v, u * w);
       z3[i] = mkc(v * w, (T)0);
     }
-    fft_exec<E, false>(z1, lds, f.prow, f.n, f.tw, lay, tid, NT);
+    fft_exec<E, false>(z1, lds, f.prow, f.twp, lay, tid, NT);
     line_store_pair<T, E>(z1, f, lds, 0, 1, row0, nrows, tid, NT, lay);
-    fft_exec<E, false>(z2, lds, f.prow, f.n, f.tw, lay, tid, NT);
+    fft_exec<E, false>(z2, lds, f.prow, f.twp, lay, tid, NT);
     line_store_pair<T, E>(z2, f, lds, 2, 3, row0, nrows, tid, NT, lay);
-    fft_exec<E, false>(z3, lds, f.prow, f.n, f.tw, lay, tid, NT);
+    fft_exec<E, false>(z3, lds, f.prow, f.twp, lay, tid, NT);
     line_store_real<T, E>(z3, f, 4, row0, nrows, tid, NT, lay);
   } else {
     cx<T> z1[E], z2[E], z3[E];
     line_load_pair<T, E>(z1, f, 0, 1, row0, nrows, tid, NT, lay);
-    fft_exec<E, true>(z1, lds, f.prow, f.n, f.tw, lay, tid, NT);
+    fft_exec<E, true>(z1, lds, f.prow, f.twp, lay, tid, NT);
     line_load_pair<T, E>(z2, f, 2, 3, row0, nrows, tid, NT, lay);
-    fft_exec<E, true>(z2, lds, f.prow, f.n, f.tw, lay, tid, NT);
+    fft_exec<E, true>(z2, lds, f.prow, f.twp, lay, tid, NT);
     line_load_pair<T, E>(z3, f, 4, 5, row0, nrows, tid, NT, lay);
-    fft_exec<E, true>(z3, lds, f.prow, f.n, f.tw, lay, tid, NT);
+    fft_exec<E, true>(z3, lds, f.prow, f.twp, lay, tid, NT);
     if (want_max) {
       double m[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -184,16 +184,16 @@ __global__ void __launch_bounds__(512) k_line3d(Flow<T> f, int flags) {
       z1[i] = mkc(v * oz - w * oy, w * ox - u * oz);
       z2[i] = mkc(u * oy - v * ox, (T)0);
     }
-    fft_exec<E, false>(z1, lds, f.prow, f.n, f.tw, lay, tid, NT);
+    fft_exec<E, false>(z1, lds, f.prow, f.twp, lay, tid, NT);
     line_store_pair<T, E>(z1, f, lds, 0, 1, row0, nrows, tid, NT, lay);
-    fft_exec<E, false>(z2, lds, f.prow, f.n, f.tw, lay, tid, NT);
+    fft_exec<E, false>(z2, lds, f.prow, f.twp, lay, tid, NT);
     line_store_real<T, E>(z2, f, 2, row0, nrows, tid, NT, lay);
   }
 }
 
 // Middle axis (y) of nfld fields: for fixed x, column tile j0..j0+yT, rows x n + y.
 template <typename T, int E, bool INV>
-__global__ void __launch_bounds__(512) k_ypass3d(Flow<T> f, int nfld) {
+__global__ void __launch_bounds__(512, 1) k_ypass3d(Flow<T> f, int nfld) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int tid = threadIdx.x, NT = blockDim.x;
@@ -215,7 +215,7 @@ __global__ void __launch_bounds__(512) k_ypass3d(Flow<T> f, int nfld) {
         v[i] = Wf[(rb + e) * f.ldw + j0 + t];
       }
     }
-    fft_exec<E, INV>(v, lds, f.py, f.n, f.tw, lay, tid, NT);
+    fft_exec<E, INV>(v, lds, f.py, f.twp, lay, tid, NT);
     {
       const int R = last_radix(f.py, INV);
 #pragma unroll
@@ -231,7 +231,7 @@ __global__ void __launch_bounds__(512) k_ypass3d(Flow<T> f, int nfld) {
 // Outer axis (x) with the spectral physics: for fixed y-wavenumber index, column tile
 // j0..j0+colT, rows x n + ky_idx.
 template <typename T, int E>
-__global__ void __launch_bounds__(512) k_spec3d(Flow<T> f, int stage) {
+__global__ void __launch_bounds__(512, 1) k_spec3d(Flow<T> f, int stage) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int tid = threadIdx.x, NT = blockDim.x;
@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(512) k_spec3d(Flow<T> f, int stage) {
           v[i] = Wf[((long long)e * n + iy) * f.ldw + j0 + t];
         }
       }
-      fft_exec<E, false>(v, lds, f.pcol, n, f.tw, lay, tid, NT);
+      fft_exec<E, false>(v, lds, f.pcol, f.twp, lay, tid, NT);
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         if (f.rot) {
@@ -357,7 +357,7 @@ __global__ void __launch_bounds__(512) k_spec3d(Flow<T> f, int stage) {
         else v[i] = mul_i(csub(cscl(fx, a1[i]), cscl(fy, a0[i])));
       }
     }
-    fft_exec<E, true>(v, lds, f.pcol, n, f.tw, lay, tid, NT);
+    fft_exec<E, true>(v, lds, f.pcol, f.twp, lay, tid, NT);
     cx<T>* Wf = f.W + c * f.fstride;
     const int Ri = last_radix(f.pcol, true);
 #pragma unroll
@@ -374,12 +374,12 @@ template <typename T>
 hipError_t launch_line3d(const Flow<T>& f, const Launch& lc, int flags, hipStream_t st) {
   switch (lc.E) {
 #define PSX_CASE(EV)                                                                 \
-  case EV: {                                                                         \
+  case EV: if constexpr (e_ok_3d(EV)) {                                                                         \
     auto fn = k_line3d<T, EV>;                                                       \
     static const bool attr_ = set_lds_max((const void*)fn); (void)attr_;                                          \
     hipLaunchKernelGGL(fn, dim3(lc.blocks), dim3(lc.threads), lc.lds, st, f, flags); \
     break;                                                                           \
-  }
+  } else { return hipErrorInvalidValue; }
     PSX_E_CASES
 #undef PSX_CASE
     default:
@@ -393,7 +393,7 @@ hipError_t launch_ypass3d(const Flow<T>& f, const Launch& lc, bool inv, int nfld
                           hipStream_t st) {
   switch (lc.E) {
 #define PSX_CASE(EV)                                                                  \
-  case EV: {                                                                          \
+  case EV: if constexpr (e_ok_3d(EV)) {                                                                          \
     if (inv) {                                                                        \
       auto fn = k_ypass3d<T, EV, true>;                                               \
       static const bool attr_ = set_lds_max((const void*)fn); (void)attr_;                                         \
@@ -404,7 +404,7 @@ hipError_t launch_ypass3d(const Flow<T>& f, const Launch& lc, bool inv, int nfld
       hipLaunchKernelGGL(fn, dim3(lc.blocks), dim3(lc.threads), lc.lds, st, f, nfld); \
     }                                                                                 \
     break;                                                                            \
-  }
+  } else { return hipErrorInvalidValue; }
     PSX_E_CASES
 #undef PSX_CASE
     default:
@@ -417,12 +417,12 @@ template <typename T>
 hipError_t launch_spec3d(const Flow<T>& f, const Launch& lc, int stage, hipStream_t st) {
   switch (lc.E) {
 #define PSX_CASE(EV)                                                                 \
-  case EV: {                                                                         \
+  case EV: if constexpr (e_ok_3d(EV)) {                                                                         \
     auto fn = k_spec3d<T, EV>;                                                       \
     static const bool attr_ = set_lds_max((const void*)fn); (void)attr_;                                          \
     hipLaunchKernelGGL(fn, dim3(lc.blocks), dim3(lc.threads), lc.lds, st, f, stage); \
     break;                                                                           \
-  }
+  } else { return hipErrorInvalidValue; }
     PSX_E_CASES
 #undef PSX_CASE
     default:

@@ -24,7 +24,13 @@ inline bool set_lds_max(const void* fn) {
          hipSuccess;
 }
 
-// Elements-per-thread variants compiled for every transform kernel.
+// Elements-per-thread variants compiled for the transform kernels (the planner in psx_api.cpp
+// only picks these): 2D fp32 {8,16,24,32}, fp64 {8,12,16}; 3D {8,12,16}.
+template <typename T>
+constexpr bool e_ok_2d(int E) {
+  return sizeof(T) == 4 ? (E == 8 || E == 16 || E == 24 || E == 32) : (E == 8 || E == 12 || E == 16);
+}
+constexpr bool e_ok_3d(int E) { return E == 8 || E == 12 || E == 16; }
 #define PSX_E_CASES PSX_CASE(8) PSX_CASE(12) PSX_CASE(16) PSX_CASE(24) PSX_CASE(32)
 
 template <typename T>
@@ -42,6 +48,11 @@ hipError_t launch_ypass3d(const Flow<T>& f, const Launch& lc, bool inv, int nfld
                           hipStream_t st);
 template <typename T>
 hipError_t launch_spec3d(const Flow<T>& f, const Launch& lc, int stage, hipStream_t st);
+
+// Compile-time specialised 2D fp32 kernels (psx_k2d_ct.hip).  which: 0 line (arg = flags),
+// 1 spectral (arg = stage), 2 outer forward, 3 outer inverse.
+bool ct2d_supported(int n);
+hipError_t launch_ct2d(const Flow<float>& f, int which, int arg, hipStream_t st);
 
 // Utility / diagnostics kernels (psx_kutil.hip).
 hipError_t launch_params(double* prm, int nh, double nu, double dt, double alpha, double amp,

@@ -58,6 +58,29 @@ def test_layout_query(n, dim, form, fp64, ok):
         assert lib.psx_last_error()
 
 
+@pytest.mark.skipif(not _lib.LIB_PATH.exists(), reason="libpsx.so not built")
+@pytest.mark.parametrize("n", [8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768,
+                               1024, 1536, 2048, 4096, 8192])
+@pytest.mark.parametrize("dim,form", [(2, 0), (3, 0), (3, 1)])
+@pytest.mark.parametrize("fp64", [0, 1])
+def test_planner_picks_compiled_variants(n, dim, form, fp64):
+    """Every launch shape the planner picks exists in the compiled kernel set and fits a
+    work-group (the launchers reject other shapes with an error, never a bad launch)."""
+    if dim == 3 and n > 512:
+        pytest.skip("3D grids above 512 are not planned")
+    lib = _lib.load()
+    p = _lib.FlowParams(n=n, dim=dim, form=form, fp64=fp64, nu=1e-3, alpha=0.0, ka2=1e30,
+                        klo2=0.0, khi2=0.0)
+    lay = _lib.Layout()
+    assert lib.psx_query_layout(ctypes.byref(p), ctypes.byref(lay)) == 0, lib.psx_last_error()
+    ok2d = {8, 16, 24, 32} if not fp64 else {8, 12, 16}
+    ok = ok2d if dim == 2 else {8, 12, 16}
+    for e, t in ((lay.e_line, lay.t_line), (lay.e_spec, lay.t_spec),
+                 (lay.e_outer, lay.t_outer), (lay.e_mid, lay.t_mid)):
+        if e:
+            assert e in ok and 1 <= t <= 512
+
+
 def test_registry_surface():
     assert names() == ["hip/fused", "hip/fused-graph"]
     assert get("hip").variant == "fused"

@@ -72,8 +72,10 @@ def test_tgv2d_exact(name):
 
 
 @pytest.mark.parametrize("n,dtype,steps", [(96, "float64", 3), (128, "float32", 3),
-                                           (384, "float32", 2), (1024, "float32", 2),
-                                           (1024, "float64", 2), (2048, "float32", 1)])
+                                           (256, "float64", 2), (384, "float32", 2),
+                                           (512, "float32", 2), (1024, "float32", 2),
+                                           (1024, "float64", 2), (2048, "float32", 1),
+                                           (4096, "float32", 1)])
 def test_2d_sizes_and_four_step(n, dtype, steps):
     print(f"case 2D n={n} {dtype}", flush=True)
     cfg = Config(n=n, dim=2, nu=1e-3, alpha=0.02, force_amp=0.2, force_klo=2, force_khi=4,
@@ -153,7 +155,8 @@ def test_graph_replays_match_eager():
 
 def test_tgv2d_full_size_exact_decay():
     """8192^2 fp32: the Taylor-Green vortex is an exact solution at every n (size-independent
-    property of the full-size grid)."""
+    property of the full-size grid).  Tolerance: the 1e-5 relative bound set for the 8192^2
+    fp32 step against the FP64 reference."""
     n, nu = 8192, 1e-2
     s = _make("hip/fused-graph", Config(n=n, dim=2, nu=nu, dtype="float32"))
     w0 = initial.to_spectral(initial.taylor_green_2d(n), 2)
@@ -161,4 +164,6 @@ def test_tgv2d_full_size_exact_decay():
     s.advance(5, 1e-3)
     got = s.get_state()
     exact = w0 * math.exp(-2 * nu * s.t)
-    assert O.rel_err(got, exact) < 5e-6
+    err = O.rel_err(got, exact)
+    print(f"  8192^2 TGV rel err {err:.3e}", flush=True)
+    assert err < 1e-5
