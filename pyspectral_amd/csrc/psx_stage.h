@@ -82,6 +82,48 @@ __device__ __forceinline__ cx<T> rk_update(int stage, T dt, T E, T Eh, cx<T> N,
   return r;
 }
 
+// Value form of rk_update: u is the stage's state value, acc the accumulator (updated in
+// place); returns the next stage input (stage 3: the new state).
+template <int stage, typename T>
+__device__ __forceinline__ cx<T> rk_value(T dt, T E, T Eh, cx<T> N, cx<T> u, cx<T>& acc) {
+  if constexpr (stage == 0) {
+    acc = cscl(E, N);
+    return cscl(Eh, cadd(u, cscl((T)0.5 * dt, N)));
+  } else if constexpr (stage == 1) {
+    acc = cadd(acc, cscl((T)2 * Eh, N));
+    return cadd(cscl(Eh, u), cscl((T)0.5 * dt, N));
+  } else if constexpr (stage == 2) {
+    acc = cadd(acc, cscl((T)2 * Eh, N));
+    return cadd(cscl(E, u), cscl(dt * Eh, N));
+  } else {
+    return cadd(cscl(E, u), cscl(dt / (T)6, cadd(acc, N)));
+  }
+}
+
+// Raw buffer access to fp32 complex data: loads at out-of-range byte offsets return zero and
+// stores there are dropped, so masked accesses need no branches.  The region must be smaller
+// than 2 GiB (kOob lies beyond it).  Each complex is moved as two 32-bit halves, which the
+// compiler merges into one 64-bit access (the 64-bit raw buffer load builtin of this toolchain
+// loads only the first dword).
+constexpr unsigned kOob = 0x80000000u;
+
+struct BufC32 {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ BufC32(const void* base, unsigned long long bytes)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0,
+                                            (int)(bytes < kOob ? bytes : kOob - 8), 0x00020000)) {}
+  __device__ __forceinline__ cx<float> load(unsigned off) const {
+    const unsigned a = __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
+    const unsigned b = __builtin_amdgcn_raw_buffer_load_b32(r, (int)(off + 4u), 0, 0);
+    return mkc(__builtin_bit_cast(float, a), __builtin_bit_cast(float, b));
+  }
+  __device__ __forceinline__ void store(unsigned off, cx<float> x) const {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x.x), r, (int)off, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x.y), r, (int)(off + 4u),
+                                          0, 0);
+  }
+};
+
 // Block-wide max of NV non-negative values, folded into stats[] with one atomic per
 // work-group (bit patterns of non-negative doubles order like unsigned 64-bit integers).
 // Works for any block size up to 512, including partial wavefronts.  `red` is LDS scratch of
