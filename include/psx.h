@@ -44,7 +44,7 @@ typedef struct psx_flow_params {
 
 /* Buffer sizes in complex elements of the working precision. */
 typedef struct psx_layout {
-  long long state_elems; /* u0 and acc each: n^(d-1) (n/2+1) (x3 in 3D), NumPy r2c layout */
+  long long state_elems; /* u0 and acc each: rows * state_pitch, rows = n^(d-1) (x3 in 3D) */
   long long work_elems;  /* work buffer: nfields * n^(d-1) * ldw */
   int nfields;           /* work fields: 2 (2D), 5 (3D basdevant), 6 (3D rotational) */
   int ldw;               /* padded retained half-axis width of a work row */
@@ -52,6 +52,8 @@ typedef struct psx_layout {
   int four_step;         /* 1 if the full axes use the two-kernel split */
   int e_line, e_spec, e_outer, e_mid; /* elements per thread of each kernel (0: unused) */
   int t_line, t_spec, t_outer, t_mid; /* threads per work-group of each kernel */
+  int state_pitch;       /* row pitch of u0/acc in complex elements (>= n/2+1, 128-byte rows);
+                            columns 0..n/2 of each row are the NumPy rfftn coefficients */
 } psx_layout;
 
 /* Plan the kernels for p and report the buffer sizes.  Replaces the buffer and plan set-up
@@ -109,14 +111,16 @@ int psx_timing_end(psx_ctx* c, void* stream, double* ms, int* count);
 int psx_velocity_max(psx_ctx* c, double* out3, void* stream);
 
 /* FP64 energy, enstrophy, palinstrophy and band energy (klo^2 <= k^2 <= khi^2) of a state
- * in NumPy layout with this context's size and precision; synchronises the stream.
+ * with this context's size and precision, rows of n/2+1 coefficients `pitch` elements apart
+ * (components n^(d-1) rows apart); synchronises the stream.
  * Replaces src/pyspectral/diagnostics.py:65-88. */
-int psx_reduce(psx_ctx* c, const void* state, double klo2, double khi2, double* out4,
-               void* stream);
+int psx_reduce(psx_ctx* c, const void* state, int pitch, double klo2, double khi2,
+               double* out4, void* stream);
 
-/* FP64 shell spectrum E(k), k = 0..nshell-1 (shell = int(|k| + 0.5)); synchronises.
- * Replaces src/pyspectral/diagnostics.py:90-97. */
-int psx_spectrum(psx_ctx* c, const void* state, double* out, int nshell, void* stream);
+/* FP64 shell spectrum E(k), k = 0..nshell-1 (shell = int(|k| + 0.5)), same state layout as
+ * psx_reduce; synchronises.  Replaces src/pyspectral/diagnostics.py:90-97. */
+int psx_spectrum(psx_ctx* c, const void* state, int pitch, double* out, int nshell,
+                 void* stream);
 
 /* Message of the last failure on this thread. */
 const char* psx_last_error(void);

@@ -23,7 +23,8 @@ class Layout(ctypes.Structure):
     _fields_ = [("state_elems", c_longlong), ("work_elems", c_longlong), ("nfields", c_int),
                 ("ldw", c_int), ("jm", c_int), ("four_step", c_int),
                 ("e_line", c_int), ("e_spec", c_int), ("e_outer", c_int), ("e_mid", c_int),
-                ("t_line", c_int), ("t_spec", c_int), ("t_outer", c_int), ("t_mid", c_int)]
+                ("t_line", c_int), ("t_spec", c_int), ("t_outer", c_int), ("t_mid", c_int),
+                ("state_pitch", c_int)]
 
 
 # name -> (restype, argtypes); the exported C ABI
@@ -37,9 +38,9 @@ SIGNATURES = {
     "psx_step": (c_int, [c_void_p, c_void_p]),
     "psx_step_graph": (c_int, [c_void_p, c_void_p]),
     "psx_velocity_max": (c_int, [c_void_p, POINTER(c_double), c_void_p]),
-    "psx_reduce": (c_int, [c_void_p, c_void_p, c_double, c_double, POINTER(c_double),
+    "psx_reduce": (c_int, [c_void_p, c_void_p, c_int, c_double, c_double, POINTER(c_double),
                            c_void_p]),
-    "psx_spectrum": (c_int, [c_void_p, c_void_p, POINTER(c_double), c_int, c_void_p]),
+    "psx_spectrum": (c_int, [c_void_p, c_void_p, c_int, POINTER(c_double), c_int, c_void_p]),
     "psx_timing_begin": (c_int, [c_void_p]),
     "psx_timing_end": (c_int, [c_void_p, c_void_p, POINTER(c_double), POINTER(c_int)]),
     "psx_last_error": (c_char_p, []),

@@ -39,11 +39,15 @@ class _Buffers:
         if lib.psx_query_layout(ctypes.byref(params), ctypes.byref(lay)) != 0:
             raise BackendUnavailable(lib.psx_last_error().decode())
         self.layout = lay
-        self.u0 = torch.zeros(state_shape, dtype=cdt, device="cuda")
-        self.acc = torch.zeros_like(self.u0)
+        # state rows padded to whole 128-byte lines; the solver exposes the NumPy-layout view
+        padded = state_shape[:-1] + (lay.state_pitch,)
+        self.u0_rows = torch.zeros(padded, dtype=cdt, device="cuda")
+        self.acc = torch.zeros(padded, dtype=cdt, device="cuda")
+        self.u0 = self.u0_rows[..., : state_shape[-1]]
+        self.pitch = lay.state_pitch
         self.work = torch.zeros(lay.work_elems, dtype=cdt, device="cuda")
         ctx = ctypes.c_void_p()
-        _lib.check(lib.psx_create(ctypes.byref(params), ctypes.c_void_p(self.u0.data_ptr()),
+        _lib.check(lib.psx_create(ctypes.byref(params), ctypes.c_void_p(self.u0_rows.data_ptr()),
                                   ctypes.c_void_p(self.acc.data_ptr()),
                                   ctypes.c_void_p(self.work.data_ptr()), ctypes.byref(ctx)))
         self.ctx = ctx

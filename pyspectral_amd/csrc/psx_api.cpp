@@ -197,7 +197,8 @@ int plan_flow(const psx_flow_params& p, psx_ctx* c, Flow<T>* f) {
   int nfields = 2;
   if (p.dim == 3) nfields = f->rot ? 6 : 5;
   f->fstride = (long long)nrows * f->ldw;
-  f->sstride = (long long)nrows * f->nh;
+  f->sp = ct_state_pitch(n, fp64);
+  f->sstride = (long long)nrows * f->sp;
 
   const int nmax = p.dim == 2 ? 2 : 3;
   if (p.dim == 2) {
@@ -250,6 +251,7 @@ int plan_flow(const psx_flow_params& p, psx_ctx* c, Flow<T>* f) {
     c->yI = c->yF;
   }
   c->lay.state_elems = (long long)(p.dim == 2 ? 1 : 3) * f->sstride;
+  c->lay.state_pitch = f->sp;
   c->lay.work_elems = (long long)nfields * f->fstride;
   c->lay.nfields = nfields;
   c->lay.ldw = f->ldw;
@@ -632,36 +634,39 @@ int psx_velocity_max(psx_ctx* c, double* out3, void* stream) {
   return c->fp64 ? velocity_max<double>(c, out3, st) : velocity_max<float>(c, out3, st);
 }
 
-int psx_reduce(psx_ctx* c, const void* state, double klo2, double khi2, double* out4,
-               void* stream) {
+int psx_reduce(psx_ctx* c, const void* state, int pitch, double klo2, double khi2,
+               double* out4, void* stream) {
   if (!c || !state || !out4) return fail("psx_reduce: null argument");
-  hipStream_t st = static_cast<hipStream_t>(stream);
   const int n = c->p.n, dim = c->p.dim;
-  const long long sst = (long long)(dim == 2 ? n : n * n) * (n / 2 + 1);
+  if (pitch < n / 2 + 1) return fail("psx_reduce: pitch %d < n/2+1", pitch);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const long long sst = (long long)(dim == 2 ? n : n * n) * pitch;
   if (c->fp64)
-    PSX_HIP(launch_reduce<double>(static_cast<const cx<double>*>(state), sst, n, dim, klo2,
-                                  khi2, c->partial, c->nred, c->dout, st));
+    PSX_HIP(launch_reduce<double>(static_cast<const cx<double>*>(state), sst, pitch, n, dim,
+                                  klo2, khi2, c->partial, c->nred, c->dout, st));
   else
-    PSX_HIP(launch_reduce<float>(static_cast<const cx<float>*>(state), sst, n, dim, klo2, khi2,
-                                 c->partial, c->nred, c->dout, st));
+    PSX_HIP(launch_reduce<float>(static_cast<const cx<float>*>(state), sst, pitch, n, dim, klo2,
+                                 khi2, c->partial, c->nred, c->dout, st));
   PSX_HIP(hipMemcpyAsync(out4, c->dout, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
   PSX_HIP(hipStreamSynchronize(st));
   return 0;
 }
 
-int psx_spectrum(psx_ctx* c, const void* state, double* out, int nshell, void* stream) {
+int psx_spectrum(psx_ctx* c, const void* state, int pitch, double* out, int nshell,
+                 void* stream) {
   if (!c || !state || !out) return fail("psx_spectrum: null argument");
   if (nshell < 1 || nshell > c->nshell_max)
     return fail("psx_spectrum: nshell=%d outside [1, %d]", nshell, c->nshell_max);
-  hipStream_t st = static_cast<hipStream_t>(stream);
   const int n = c->p.n, dim = c->p.dim;
-  const long long sst = (long long)(dim == 2 ? n : n * n) * (n / 2 + 1);
+  if (pitch < n / 2 + 1) return fail("psx_spectrum: pitch %d < n/2+1", pitch);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const long long sst = (long long)(dim == 2 ? n : n * n) * pitch;
   if (c->fp64)
-    PSX_HIP(launch_spectrum<double>(static_cast<const cx<double>*>(state), sst, n, dim,
+    PSX_HIP(launch_spectrum<double>(static_cast<const cx<double>*>(state), sst, pitch, n, dim,
                                     c->dout, nshell, st));
   else
-    PSX_HIP(launch_spectrum<float>(static_cast<const cx<float>*>(state), sst, n, dim, c->dout,
-                                   nshell, st));
+    PSX_HIP(launch_spectrum<float>(static_cast<const cx<float>*>(state), sst, pitch, n, dim,
+                                   c->dout, nshell, st));
   PSX_HIP(hipMemcpyAsync(out, c->dout, nshell * sizeof(double), hipMemcpyDeviceToHost, st));
   PSX_HIP(hipStreamSynchronize(st));
   return 0;

@@ -1,8 +1,10 @@
 // Flow descriptor shared by the host planner and the stage kernels.
 //
 // HBM layout of one flow (all complex fields in the working precision T):
-//   u0, acc   spectral state and RK accumulator, NumPy r2c layout: 2D (n, nh), 3D (3, n, n, nh),
-//             nh = n/2 + 1.  Only the retained 2/3-rule modes are ever read or written.
+//   u0, acc   spectral state and RK accumulator in r2c layout with a padded row pitch sp >= nh
+//             (nh = n/2 + 1): element (.., row, j) at row * sp + j, 2D rows n, 3D rows 3 n^2;
+//             the first nh columns of each row are the NumPy rfftn coefficients.  Only the
+//             retained 2/3-rule modes are ever read or written.
 //   W         work fields, each of nrows x ldw complex, nrows = n^(d-1).  Column index j is the
 //             half-axis wavenumber and only j < jm (the retained ones, 3 j < n) are stored;
 //             ldw pads jm to a whole number of column tiles.  The same field holds, per stage,
@@ -35,6 +37,7 @@ struct Flow {
   long long fstride; // complex elements per work field (nrows * ldw)
   long long sstride; // complex elements per state component (3D)
   int n, nh, jm, ldw;
+  int sp;            // state row pitch (complex elements)
   int dim, rot;      // rot: 3D rotational form
   int n1, n2;        // four-step split of the full axes (n1 = 1: single pass)
   int rowT;          // rows per work-group in the line kernel

@@ -67,12 +67,18 @@ constexpr int ct_twoff(const CtPlan& p, bool inv, int q) {
 constexpr int ct_twsize(const CtPlan& p, bool inv) { return ct_twoff(p, inv, p.np); }
 
 constexpr int ct_rows_pitch(int L) { return L + (L >> 4) + 1; }
+// Row pitch of the state (complex elements): n/2+1 rounded up to whole 128-byte lines.
+constexpr int ct_state_pitch(int n, bool fp64) {
+  const int q = fp64 ? 8 : 16;
+  return (n / 2 + 1 + q - 1) / q * q;
+}
 constexpr int ct_retained(int n) { return (n - 1) / 3 + 1; }
 
 // Launch geometry of the 2D kernels for grid size n in fp32 (csz = 8) or fp64 (csz = 16).
 struct Geo2D {
   bool ok = false;
   int n = 0, jm = 0, colT = 0, ldw = 0, n1 = 1, n2 = 0;
+  int sp = 0;  // state row pitch: n/2+1 rounded up to whole 128-byte lines
   int lineE = 0, lineNT = 0, lineT = 0;
   int specE = 0, specNT = 0;
   int outE = 0, outNT = 0;
@@ -89,13 +95,14 @@ constexpr Geo2D geo2d(int n, bool fp64) {
   const int csz = fp64 ? 16 : 8;
   g.n = n;
   g.jm = ct_retained(n);
+  g.sp = ct_state_pitch(n, fp64);
   int T = fp64 ? 8 : 16;
   while (T > 1 && T / 2 >= g.jm) T /= 2;
   g.colT = T;
   g.ldw = (g.jm + T - 1) / T * T;
-  // line kernel: whole rows
+  // line kernel: whole rows (long fp32 rows: 32 elements per thread, 3 passes at 8192)
   {
-    int E = ct_pick_E(n, fp64);
+    int E = (!fp64 && n % 3 != 0 && n >= 4096) ? 32 : ct_pick_E(n, fp64);
     int t = 1;
     while (n * t / E < 128 && t * 2 <= n) t *= 2;
     int nt = n * t / E;

@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(512, 1) k_spec2d(Flow<T> f, int stage) {
       const int ax = kx < 0 ? -kx : kx;
       cx<T> ou = czero<T>(), ov = czero<T>();
       if (3 * ax < n && ky < f.jm) {
-        const long long si = (long long)k * nh + ky;
+        const long long si = (long long)k * f.sp + ky;
         const long long k2 = (long long)kx * kx + (long long)ky * ky;
         cx<T> nxt;
         if (stage == kStagePre) {

@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(G2<N>::g.specNT, 2) kc_spec2d(Flow<float> f) {
   constexpr Geo2D G = G2<N>::g;
   constexpr bool FOUR = G.n1 > 1;
   constexpr int E = G.specE, NT = G.specNT, CT = G.colT, L = G.n2, N1 = G.n1, JM = G.jm;
-  constexpr int LDW = G.ldw, NH = N / 2 + 1;
+  constexpr int LDW = G.ldw, NH = N / 2 + 1, SP = G.sp;
   constexpr long long FS = (long long)N * LDW;
   constexpr int NJT = LDW / CT;
   using Lay = ColsCT<CT>;
@@ -147,8 +147,8 @@ __global__ void __launch_bounds__(G2<N>::g.specNT, 2) kc_spec2d(Flow<float> f) {
   // State and accumulator of this thread's modes, loaded before the transforms so their
   // latency hides behind them; modes outside the 2/3 mask read zero and are never written.
   constexpr int RP = ct_radix(G.spec, false, G.spec.np - 1);
-  const BufC32 ub(f.u0, (unsigned long long)N * NH * 8);
-  const BufC32 ab(f.acc, (unsigned long long)N * NH * 8);
+  const BufC32 ub(f.u0, (unsigned long long)N * SP * 8);
+  const BufC32 ab(f.acc, (unsigned long long)N * SP * 8);
   unsigned soff[E];
   cx<T> uu[E], aa[E];
 #pragma unroll
@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(G2<N>::g.specNT, 2) kc_spec2d(Flow<float> f) {
     const int kx = k < N / 2 ? k : k - N;
     const int ax = kx < 0 ? -kx : kx;
     const int ky = j0 + t;
-    soff[i] = (3 * ax < N && ky < JM) ? (unsigned)(k * NH + ky) * 8u : kOob;
+    soff[i] = (3 * ax < N && ky < JM) ? (unsigned)(k * SP + ky) * 8u : kOob;
     uu[i] = ub.load(soff[i]);
   }
   if constexpr (stage >= 1) {

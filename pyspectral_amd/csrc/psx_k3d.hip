@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(512, 1) k_spec3d(Flow<T> f, int stage) {
       const int kz = j0 + t;
       cx<T> ua = czero<T>(), ub = czero<T>(), uc = czero<T>();
       if (3 * ax < n && 3 * ay < n && kz < f.jm) {
-        const long long si = ((long long)e * n + iy) * nh + kz;
+        const long long si = ((long long)e * n + iy) * f.sp + kz;
         const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
         if (stage == kStagePre) {
           ua = f.u0[si];

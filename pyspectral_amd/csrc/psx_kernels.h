@@ -60,11 +60,11 @@ hipError_t launch_params(double* prm, int nh, double nu, double dt, double alpha
 hipError_t launch_fill_u32(unsigned* p, unsigned v, int n, hipStream_t st);
 hipError_t launch_fill_f64(double* p, double v, int n, hipStream_t st);
 template <typename T>
-hipError_t launch_reduce(const cx<T>* state, long long sstride, int n, int dim, double klo2,
-                         double khi2, double* partial, int nblocks, double* out,
+hipError_t launch_reduce(const cx<T>* state, long long sstride, int pitch, int n, int dim,
+                         double klo2, double khi2, double* partial, int nblocks, double* out,
                          hipStream_t st);
 template <typename T>
-hipError_t launch_spectrum(const cx<T>* state, long long sstride, int n, int dim,
+hipError_t launch_spectrum(const cx<T>* state, long long sstride, int pitch, int n, int dim,
                            double* out, int nshell, hipStream_t st);
 
 }  // namespace psx

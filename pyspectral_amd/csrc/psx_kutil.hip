@@ -77,8 +77,8 @@ constexpr int kRedThreads = 256;
 
 template <typename T>
 __global__ void __launch_bounds__(kRedThreads)
-    k_reduce(const cx<T>* __restrict__ st, long long sstride, int n, int dim, double klo2,
-             double khi2, double* __restrict__ partial) {
+    k_reduce(const cx<T>* __restrict__ st, long long sstride, int pitch, int n, int dim,
+             double klo2, double khi2, double* __restrict__ partial) {
   const int nh = n / 2 + 1;
   const long long nm = (dim == 2 ? (long long)n : (long long)n * n) * nh;
   const double inv_norm = 1.0 / pow((double)n, 2.0 * dim);
@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(kRedThreads)
     const int j = (int)(idx % nh);
     double a = 0.0;
     for (int c = 0; c < (dim == 2 ? 1 : 3); ++c) {
-      const cx<T> z = st[c * sstride + idx];
+      const cx<T> z = st[c * sstride + (idx / nh) * pitch + j];
       a += (double)z.x * (double)z.x + (double)z.y * (double)z.y;
     }
     const double w = (j == 0 || j == nh - 1) ? 1.0 : 2.0;
@@ -144,18 +144,18 @@ __global__ void k_reduce_final(const double* __restrict__ partial, int nblocks, 
 }
 
 template <typename T>
-hipError_t launch_reduce(const cx<T>* state, long long sstride, int n, int dim, double klo2,
-                         double khi2, double* partial, int nblocks, double* out,
+hipError_t launch_reduce(const cx<T>* state, long long sstride, int pitch, int n, int dim,
+                         double klo2, double khi2, double* partial, int nblocks, double* out,
                          hipStream_t st) {
-  hipLaunchKernelGGL(k_reduce<T>, dim3(nblocks), dim3(kRedThreads), 0, st, state, sstride, n,
-                     dim, klo2, khi2, partial);
+  hipLaunchKernelGGL(k_reduce<T>, dim3(nblocks), dim3(kRedThreads), 0, st, state, sstride,
+                     pitch, n, dim, klo2, khi2, partial);
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedThreads), 0, st, partial, nblocks, out);
   return hipGetLastError();
 }
 
 template <typename T>
 __global__ void __launch_bounds__(kRedThreads)
-    k_spectrum(const cx<T>* __restrict__ st, long long sstride, int n, int dim,
+    k_spectrum(const cx<T>* __restrict__ st, long long sstride, int pitch, int n, int dim,
                double* __restrict__ out, int nshell) {
   extern __shared__ double hist[];
   for (int s = threadIdx.x; s < nshell; s += blockDim.x) hist[s] = 0.0;
@@ -170,7 +170,7 @@ __global__ void __launch_bounds__(kRedThreads)
     const int j = (int)(idx % nh);
     double a = 0.0;
     for (int c = 0; c < (dim == 2 ? 1 : 3); ++c) {
-      const cx<T> z = st[c * sstride + idx];
+      const cx<T> z = st[c * sstride + (idx / nh) * pitch + j];
       a += (double)z.x * (double)z.x + (double)z.y * (double)z.y;
     }
     const double w = (j == 0 || j == nh - 1) ? 1.0 : 2.0;
@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(kRedThreads)
 }
 
 template <typename T>
-hipError_t launch_spectrum(const cx<T>* state, long long sstride, int n, int dim,
+hipError_t launch_spectrum(const cx<T>* state, long long sstride, int pitch, int n, int dim,
                            double* out, int nshell, hipStream_t st) {
   hipError_t e = launch_fill_f64(out, 0.0, nshell, st);
   if (e != hipSuccess) return e;
@@ -194,18 +194,18 @@ hipError_t launch_spectrum(const cx<T>* state, long long sstride, int n, int dim
   if (lds > 65536)
     (void)hipFuncSetAttribute((const void*)k_spectrum<T>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL(k_spectrum<T>, dim3(1024), dim3(kRedThreads), lds, st, state, sstride, n,
-                     dim, out, nshell);
+  hipLaunchKernelGGL(k_spectrum<T>, dim3(1024), dim3(kRedThreads), lds, st, state, sstride,
+                     pitch, n, dim, out, nshell);
   return hipGetLastError();
 }
 
-template hipError_t launch_reduce<float>(const cx<float>*, long long, int, int, double, double,
-                                         double*, int, double*, hipStream_t);
-template hipError_t launch_reduce<double>(const cx<double>*, long long, int, int, double,
+template hipError_t launch_reduce<float>(const cx<float>*, long long, int, int, int, double,
+                                         double, double*, int, double*, hipStream_t);
+template hipError_t launch_reduce<double>(const cx<double>*, long long, int, int, int, double,
                                           double, double*, int, double*, hipStream_t);
-template hipError_t launch_spectrum<float>(const cx<float>*, long long, int, int, double*, int,
-                                           hipStream_t);
-template hipError_t launch_spectrum<double>(const cx<double>*, long long, int, int, double*,
-                                            int, hipStream_t);
+template hipError_t launch_spectrum<float>(const cx<float>*, long long, int, int, int, double*,
+                                           int, hipStream_t);
+template hipError_t launch_spectrum<double>(const cx<double>*, long long, int, int, int,
+                                            double*, int, hipStream_t);
 
 }  // namespace psx

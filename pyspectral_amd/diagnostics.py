@@ -27,20 +27,26 @@ class DeviceDiagnostics:
         self._scratch: Any = None
 
     # ------------------------------------------------------------------ helpers
-    def _ptr(self, state: Any) -> ctypes.c_void_p:
+    def _ptr(self, state: Any) -> tuple[ctypes.c_void_p, int]:
+        """Device pointer and row pitch of a state (the solver's padded rows are used in place;
+        anything else is converted to a contiguous tensor of the working precision)."""
         s = self.s
         if not hasattr(state, "data_ptr"):
             state = s.ops.asarray(np.asarray(state), dtype=s.cdt)
         if tuple(state.shape) != s.state_shape():
             raise ValueError(f"state shape {tuple(state.shape)} != {s.state_shape()}")
+        own = s.state_native()
+        if state.data_ptr() == own.data_ptr() and state.stride() == own.stride():
+            return ctypes.c_void_p(state.data_ptr()), s.buf.pitch
         if state.dtype != s.cdt or not state.is_contiguous():
             state = state.to(s.cdt).contiguous()
         self._keep = state
-        return ctypes.c_void_p(state.data_ptr())
+        return ctypes.c_void_p(state.data_ptr()), state.shape[-1]
 
     def _reduce(self, state: Any, klo2: float = 0.0, khi2: float = -1.0) -> np.ndarray:
         out = (ctypes.c_double * 4)()
-        _lib.check(self.s.lib.psx_reduce(self.s.ctx, self._ptr(state), klo2, khi2, out,
+        ptr, pitch = self._ptr(state)
+        _lib.check(self.s.lib.psx_reduce(self.s.ctx, ptr, pitch, klo2, khi2, out,
                                          _lib.stream_ptr()))
         return np.array(out[:])
 
@@ -65,7 +71,8 @@ class DeviceDiagnostics:
 
     def spectrum(self, state: Any) -> tuple[np.ndarray, np.ndarray]:
         out = (ctypes.c_double * self.nshell)()
-        _lib.check(self.s.lib.psx_spectrum(self.s.ctx, self._ptr(state), out, self.nshell,
+        ptr, pitch = self._ptr(state)
+        _lib.check(self.s.lib.psx_spectrum(self.s.ctx, ptr, pitch, out, self.nshell,
                                            _lib.stream_ptr()))
         return np.arange(self.nshell), np.array(out[:])
 

@@ -50,7 +50,8 @@ def test_layout_query(n, dim, form, fp64, ok):
     if ok:
         nh = n // 2 + 1
         rows = n ** (dim - 1)
-        assert lay.state_elems == (1 if dim == 2 else 3) * rows * nh
+        assert lay.state_pitch >= nh and (lay.state_pitch * (16 if fp64 else 8)) % 128 == 0
+        assert lay.state_elems == (1 if dim == 2 else 3) * rows * lay.state_pitch
         assert lay.jm == grid.retained(n) and lay.ldw >= lay.jm
         assert lay.nfields == (2 if dim == 2 else (6 if form else 5))
         assert lay.work_elems == lay.nfields * rows * lay.ldw
