@@ -154,7 +154,7 @@ struct psx_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int nshell_max = 0;
-  bool ct = false;  // compile-time specialised 2D fp32 kernels for this size
+  bool ct = false;  // compile-time specialised fp32 kernels for this size (2D or 3D)
   // per-launch event timing of eager steps (psx_timing_*)
   bool timing = false;
   std::vector<hipEvent_t> events;
@@ -238,6 +238,39 @@ int plan_flow(const psx_flow_params& p, psx_ctx* c, Flow<T>* f) {
       c->outerI = c->outerF;
     }
     c->ct = !fp64 && ct2d_supported(n);
+  } else if (!fp64 && ct3d_supported(n)) {
+    // compile-time specialised 3D kernels: plans from the shared planner
+    const Geo3D g = geo3d(n);
+    if (!g.ok) return fail("psx: no 3D plan for n=%d", n);
+    auto conv = [](const CtPlan& cp, FftPlan* fp) {
+      fp->L = cp.L;
+      fp->np = cp.np;
+      for (int i = 0; i < kMaxPasses; ++i) fp->rad[i] = i < cp.np ? cp.rad[i] : 1;
+    };
+    f->colT = f->yT = g.T;
+    f->ldw = g.ldw;
+    f->fstride = (long long)nrows * f->ldw;
+    f->rowT = g.lineT;
+    f->n1 = 1;
+    f->n2 = n;
+    conv(g.line, &f->prow);
+    conv(g.yp, &f->py);
+    conv(g.xp, &f->pcol);
+    const int njt = g.ldw / g.T;
+    c->line.E = g.lineE;
+    c->line.threads = g.lineNT;
+    c->line.blocks = nrows / g.lineT;
+    c->line.lds = (size_t)g.lineT * rows_pitch(n) * csz;
+    c->yF.E = g.yE;
+    c->yF.threads = g.yNT;
+    c->yF.blocks = n * njt;
+    c->yF.lds = (size_t)n * g.T * csz;
+    c->yI = c->yF;
+    c->spec.E = g.xE;
+    c->spec.threads = g.xNT;
+    c->spec.blocks = n * njt;
+    c->spec.lds = (size_t)n * g.T * csz;
+    c->ct = true;
   } else {
     if (!plan_rows(n, nrows, fp64, true, nmax, &c->line, &f->rowT, &f->prow))
       return fail("psx: no line plan for n=%d", n);
@@ -364,6 +397,13 @@ int enqueue_prepare(psx_ctx* c, hipStream_t st) {
       PSX_HIP(launch_spec2d<T>(f, c->spec, false, kStagePre, st));
     }
   } else {
+    if constexpr (sizeof(T) == 4) {
+      if (c->ct) {
+        PSX_HIP(launch_ct3d(f, 2, kStagePre, st));
+        PSX_HIP(launch_ct3d(f, 3, 0, st));
+        return 0;
+      }
+    }
     PSX_HIP(launch_spec3d<T>(f, c->spec, kStagePre, st));
     PSX_HIP(launch_ypass3d<T>(f, c->yI, true, f.rot ? 6 : 3, st));
   }
@@ -376,7 +416,17 @@ int enqueue_step(psx_ctx* c, hipStream_t st) {
   for (int s = 0; s < 4; ++s) {
     bool done = false;
     if constexpr (sizeof(T) == 4) {
-      if (c->ct) {
+      if (c->ct && f.dim == 3) {
+        PSX_MARK(PSX_KIND_LINE);
+        PSX_HIP(launch_ct3d(f, 0, 0, st));
+        PSX_MARK(PSX_KIND_MID_FWD);
+        PSX_HIP(launch_ct3d(f, 1, 0, st));
+        PSX_MARK(PSX_KIND_SPEC);
+        PSX_HIP(launch_ct3d(f, 2, s, st));
+        PSX_MARK(PSX_KIND_MID_INV);
+        PSX_HIP(launch_ct3d(f, 3, 0, st));
+        done = true;
+      } else if (c->ct) {
         PSX_MARK(PSX_KIND_LINE);
         PSX_HIP(launch_ct2d(f, 0, 0, st));
         if (f.n1 > 1) {
@@ -429,7 +479,10 @@ int velocity_max(psx_ctx* c, double* out, hipStream_t st) {
   bool done = false;
   if constexpr (sizeof(T) == 4) {
     if (c->ct) {
-      PSX_HIP(launch_ct2d(f, 0, kLineWantMax | kLineMaxOnly, st));
+      if (f.dim == 3)
+        PSX_HIP(launch_ct3d(f, 0, kLineWantMax | kLineMaxOnly, st));
+      else
+        PSX_HIP(launch_ct2d(f, 0, kLineWantMax | kLineMaxOnly, st));
       done = true;
     }
   }

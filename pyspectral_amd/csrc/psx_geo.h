@@ -151,4 +151,53 @@ constexpr Geo2D geo2d(int n, bool fp64) {
 // Grid sizes with compile-time specialised 2D fp32 kernels.
 #define PSX_CT2D_SIZES(X) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
 
+// Launch geometry of the 3D kernels (fp32): z lines, y and x column passes over tiles of
+// T columns of the retained half axis.
+struct Geo3D {
+  bool ok = false;
+  int n = 0, jm = 0, T = 0, ldw = 0, sp = 0;
+  int lineE = 0, lineNT = 0, lineT = 0;
+  int yE = 0, yNT = 0;
+  int xE = 0, xNT = 0;
+  CtPlan line{}, yp{}, xp{};
+};
+
+constexpr Geo3D geo3d(int n) {
+  Geo3D g{};
+  g.n = n;
+  g.jm = ct_retained(n);
+  g.sp = ct_state_pitch(n, false);
+  int T = 16;
+  while (T > 1 && T / 2 >= g.jm) T /= 2;
+  g.T = T;
+  g.ldw = (g.jm + T - 1) / T * T;
+  const bool r3 = n % 3 == 0;
+  {
+    const int E = r3 ? 24 : 16;
+    int t = 1;
+    while (n * t / E < 128 && t * 2 <= n * n) t *= 2;
+    const int nt = n * t / E;
+    g.line = ct_plan(n, E);
+    if (g.line.np == 0 || nt > 512 || (n * t) % E) return g;
+    g.lineE = E;
+    g.lineNT = nt;
+    g.lineT = t;
+  }
+  auto cols = [&](int E0, int& E, int& nt, CtPlan& pl) {
+    E = E0;
+    while (E > 4 && ((n * T) % E || n * T / E > 512)) E = (E % 3 == 0 && E > 12) ? E / 2 : E;
+    pl = ct_plan(n, E);
+    nt = n * T / E;
+    return pl.np > 0 && nt >= 1 && nt <= 512 && (n * T) % E == 0;
+  };
+  if (n * T * 8 > 65536) return g;
+  if (!cols(r3 ? 24 : 16, g.yE, g.yNT, g.yp)) return g;
+  if (!cols(r3 ? 12 : 16, g.xE, g.xNT, g.xp)) return g;
+  g.ok = true;
+  return g;
+}
+
+// Grid sizes with compile-time specialised 3D fp32 kernels.
+#define PSX_CT3D_SIZES(X) X(64) X(96) X(128) X(192) X(256) X(384)
+
 }  // namespace psx

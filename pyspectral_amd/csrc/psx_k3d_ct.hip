@@ -1,0 +1,492 @@
+// 3D fp32 stage kernels specialised per grid size (PSX_CT3D_SIZES) and form: the algorithms of
+// psx_k3d.hip with plans, launch shapes and strides fixed at compile time by geo3d()
+// (psx_geo.h), branch-free masked state access, and packed line spectra at the z-axis
+// boundary (as in psx_k2d_ct.hip):
+//
+//   z-line input (written by the y inverse pass):
+//     basdevant   fields (0,1) = packed u + i v, field 2 = w_hat (one real field)
+//     rotational  fields (0,1) = packed u + i v, (2,3) = w + i omega_x, (4,5) = omega_y + i omega_z
+//   z-line output (read by the y forward pass):
+//     basdevant   (0,1) = packed T11 + i T22, (2,3) = packed T12 + i T13, field 4 = T23
+//     rotational  (0,1) = packed (u x omega)_x + i (u x omega)_y, field 2 = (u x omega)_z
+// A packed pair stores Z(j) in its first field and Z(n - j) in its second (j < jm).
+#include "psx_fft_ct.h"
+#include "psx_kernels.h"
+
+namespace psx {
+
+template <int N>
+struct G3 {
+  static constexpr Geo3D g = geo3d(N);
+  static_assert(g.ok, "no 3D plan for this size");
+  static constexpr long long NR = (long long)N * N;      // rows per field
+  static constexpr long long FS = NR * g.ldw;            // complex elements per field
+  // element offsets of all work fields stay below the in-kernel out-of-range marker, and the
+  // byte offsets below 2 GiB (raw buffer addressing)
+  static_assert(6 * FS < 0x10000000LL, "work fields too large for 32-bit buffer offsets");
+  static_assert(3 * NR * g.sp * 8 < 0x80000000LL, "state too large for 32-bit offsets");
+};
+
+__device__ __forceinline__ void z_split3(cx<float> z, cx<float> zm, cx<float>& a, cx<float>& b) {
+  const cx<float> zc = conjg(zm);
+  a = cscl(0.5f, cadd(z, zc));
+  b = cscl(0.5f, mul_mi(csub(z, zc)));
+}
+
+__device__ __forceinline__ void z_pack3(int j, cx<float> u, cx<float> v, cx<float>& z,
+                                        cx<float>& zm) {
+  if (j == 0) {
+    z = mkc(u.x, v.x);
+    zm = z;
+  } else {
+    z = mkc(u.x - v.y, u.y + v.x);
+    zm = mkc(u.x + v.y, v.x - u.y);
+  }
+}
+
+// ----------------------------------------------------------------------------- z lines
+template <int N, bool ROT>
+__global__ void __launch_bounds__(G3<N>::g.lineNT, 1) kc_line3d(Flow<float> f, int flags) {
+  using T = float;
+  constexpr Geo3D G = G3<N>::g;
+  constexpr int E = G.lineE, NT = G.lineNT, RT = G.lineT, L = N, JM = G.jm, LDW = G.ldw;
+  constexpr long long FS = G3<N>::FS;
+  constexpr int NF = ROT ? 6 : 5;
+  using Lay = RowsCT<L, RT>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const long long row0 = (long long)blockIdx.x * RT;
+  const BufC32 wb(f.W, (unsigned long long)NF * FS * 8);
+  const cx<T>* tw_fwd = f.twp + f.prow.twoff[0][0];
+  const cx<T>* tw_rev = f.twp + f.prow.twoff[1][0];
+  constexpr int RI = ct_radix(G.line, true, 0);
+  constexpr int RO = ct_radix(G.line, false, G.line.np - 1);
+
+  auto pair_off = [&](int fa, int fb, int e, int t) -> unsigned {
+    const unsigned base = (unsigned)((row0 + t) * LDW);
+    return e < JM ? (unsigned)(fa * FS) + base + e
+                  : (e > L - JM ? (unsigned)(fb * FS) + base + (L - e) : 0x10000000u);
+  };
+  auto load_pair = [&](cx<T>(&v)[E], int fa, int fb) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int e, t;
+      ct_elem<RI, L, NT, Lay>(i, tid, e, t);
+      const unsigned o = pair_off(fa, fb, e, t);
+      v[i] = wb.load(o == 0x10000000u ? kOob : o * 8u);
+    }
+  };
+  auto load_single = [&](cx<T>(&v)[E], int fa) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int e, t;
+      ct_elem<RI, L, NT, Lay>(i, tid, e, t);
+      const unsigned base = (unsigned)((row0 + t) * LDW) + (unsigned)(fa * FS);
+      const int src = e < JM ? e : (e > L - JM ? L - e : -1);
+      cx<T> z = wb.load(src >= 0 ? (base + src) * 8u : kOob);
+      if (e > L - JM) z = conjg(z);  // Hermitian extension of a real line's spectrum
+      if (e == 0) z.y = (T)0;       // c2r ignores the imaginary part of the mean
+      v[i] = z;
+    }
+  };
+  auto store_pair = [&](const cx<T>(&v)[E], int fa, int fb) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int e, t;
+      ct_elem<RO, L, NT, Lay>(i, tid, e, t);
+      const unsigned o = pair_off(fa, fb, e, t);
+      wb.store(o == 0x10000000u ? kOob : o * 8u, v[i]);
+    }
+  };
+  auto store_single = [&](const cx<T>(&v)[E], int fa) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int e, t;
+      ct_elem<RO, L, NT, Lay>(i, tid, e, t);
+      const unsigned base = (unsigned)((row0 + t) * LDW) + (unsigned)(fa * FS);
+      wb.store(e < JM ? (base + e) * 8u : kOob, v[i]);
+    }
+  };
+
+  cx<T> z1[E], z2[E];
+  load_pair(z1, 0, 1);
+  fft_ct<E, true, NT, Lay, G.line>(z1, lds, tw_rev, tid);
+  if constexpr (!ROT) {
+    load_single(z2, 2);
+    fft_ct<E, true, NT, Lay, G.line>(z2, lds, tw_rev, tid);
+    if (flags & kLineWantMax) {
+      double m[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        m[0] = fmax(m[0], abs_or_inf((double)z1[i].x));
+        m[1] = fmax(m[1], abs_or_inf((double)z1[i].y));
+        m[2] = fmax(m[2], abs_or_inf((double)z2[i].x));
+      }
+      block_max_to<3>(m, f.stats, reinterpret_cast<double*>(smem_raw));
+    }
+    if (flags & kLineMaxOnly) return;
+    cx<T> z3[E];
+    const T third = (T)1 / (T)3;
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const T u = z1[i].x, v = z1[i].y, w = z2[i].x;
+      const T q = (u * u + v * v + w * w) * third;
+      z1[i] = mkc(u * u - q, v * v - q);
+      z2[i] = mkc(u * v, u * w);
+      z3[i] = mkc(v * w, (T)0);
+    }
+    fft_ct<E, false, NT, Lay, G.line>(z1, lds, tw_fwd, tid);
+    store_pair(z1, 0, 1);
+    fft_ct<E, false, NT, Lay, G.line>(z2, lds, tw_fwd, tid);
+    store_pair(z2, 2, 3);
+    fft_ct<E, false, NT, Lay, G.line>(z3, lds, tw_fwd, tid);
+    store_single(z3, 4);
+  } else {
+    cx<T> z3[E];
+    load_pair(z2, 2, 3);
+    fft_ct<E, true, NT, Lay, G.line>(z2, lds, tw_rev, tid);
+    load_pair(z3, 4, 5);
+    fft_ct<E, true, NT, Lay, G.line>(z3, lds, tw_rev, tid);
+    if (flags & kLineWantMax) {
+      double m[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        m[0] = fmax(m[0], abs_or_inf((double)z1[i].x));
+        m[1] = fmax(m[1], abs_or_inf((double)z1[i].y));
+        m[2] = fmax(m[2], abs_or_inf((double)z2[i].x));
+      }
+      block_max_to<3>(m, f.stats, reinterpret_cast<double*>(smem_raw));
+    }
+    if (flags & kLineMaxOnly) return;
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const T u = z1[i].x, v = z1[i].y, w = z2[i].x;
+      const T ox = z2[i].y, oy = z3[i].x, oz = z3[i].y;
+      z1[i] = mkc(v * oz - w * oy, w * ox - u * oz);
+      z2[i] = mkc(u * oy - v * ox, (T)0);
+    }
+    fft_ct<E, false, NT, Lay, G.line>(z1, lds, tw_fwd, tid);
+    store_pair(z1, 0, 1);
+    fft_ct<E, false, NT, Lay, G.line>(z2, lds, tw_fwd, tid);
+    store_single(z2, 2);
+  }
+}
+
+// --------------------------------------------------------------------------- y passes
+// Tile: fixed x (group), columns j0..j0+T, rows x N + y.  Forward: split packed pairs into the
+// two real fields' spectra, transform along y.  Inverse: transform along y, pack pairs.
+template <int N, bool ROT, bool INV>
+__global__ void __launch_bounds__(G3<N>::g.yNT, 1) kc_ypass3d(Flow<float> f) {
+  using T = float;
+  constexpr Geo3D G = G3<N>::g;
+  constexpr int E = G.yE, NT = G.yNT, CT = G.T, L = N, LDW = G.ldw, NJT = LDW / CT;
+  constexpr long long FS = G3<N>::FS;
+  using Lay = ColsCT<CT>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int x = blockIdx.x / NJT;
+  const int j0 = (blockIdx.x - x * NJT) * CT;
+  const long long rb = (long long)x * N;
+  const cx<T>* tw_o = f.twp + f.py.twoff[INV ? 1 : 0][0];
+  constexpr int R0 = ct_radix(G.yp, INV, 0);
+  constexpr int RL = ct_radix(G.yp, INV, G.yp.np - 1);
+  // packed pairs and single fields of this direction
+  constexpr int NPAIR = INV ? (ROT ? 3 : 1) : (ROT ? 1 : 2);
+  constexpr int SINGLE = INV ? (ROT ? -1 : 2) : (ROT ? 2 : 4);
+
+  static_for<0, NPAIR>([&](auto pc) {
+    constexpr int fa = 2 * decltype(pc)::value, fb = fa + 1;
+    cx<T> va[E], vb[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int e, t;
+      ct_elem<R0, L, NT, Lay>(i, tid, e, t);
+      const long long a = (rb + e) * LDW + j0 + t;
+      const cx<T> z = f.W[fa * FS + a];
+      const cx<T> zm = f.W[fb * FS + a];
+      if constexpr (INV) {
+        va[i] = z;
+        vb[i] = zm;
+      } else {
+        z_split3(z, j0 + t == 0 ? z : zm, va[i], vb[i]);
+      }
+    }
+    fft_ct<E, INV, NT, Lay, G.yp>(va, lds, tw_o, tid);
+    fft_ct<E, INV, NT, Lay, G.yp>(vb, lds, tw_o, tid);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int e, t;
+      ct_elem<RL, L, NT, Lay>(i, tid, e, t);
+      const long long a = (rb + e) * LDW + j0 + t;
+      if constexpr (INV) {
+        cx<T> z, zm;
+        z_pack3(j0 + t, va[i], vb[i], z, zm);
+        f.W[fa * FS + a] = z;
+        f.W[fb * FS + a] = zm;
+      } else {
+        f.W[fa * FS + a] = va[i];
+        f.W[fb * FS + a] = vb[i];
+      }
+    }
+  });
+  if constexpr (SINGLE >= 0) {
+    cx<T> v[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int e, t;
+      ct_elem<R0, L, NT, Lay>(i, tid, e, t);
+      v[i] = f.W[SINGLE * FS + (rb + e) * LDW + j0 + t];
+    }
+    fft_ct<E, INV, NT, Lay, G.yp>(v, lds, tw_o, tid);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int e, t;
+      ct_elem<RL, L, NT, Lay>(i, tid, e, t);
+      f.W[SINGLE * FS + (rb + e) * LDW + j0 + t] = v[i];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ x pass with physics
+template <int N, bool ROT, int stage>
+__global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
+  using T = float;
+  constexpr Geo3D G = G3<N>::g;
+  constexpr int E = G.xE, NT = G.xNT, CT = G.T, L = N, JM = G.jm, LDW = G.ldw, SP = G.sp;
+  constexpr int NJT = LDW / CT, NH = N / 2 + 1;
+  constexpr long long FS = G3<N>::FS;
+  constexpr long long SST = G3<N>::NR * SP;  // complex elements per state component
+  using Lay = ColsCT<CT>;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
+  const int tid = threadIdx.x;
+  const int iy = blockIdx.x / NJT;
+  const int j0 = (blockIdx.x - iy * NJT) * CT;
+  const int ky = iy < N / 2 ? iy : iy - N;
+  const int ay = ky < 0 ? -ky : ky;
+  const cx<T>* tw_fwd = f.twp + f.pcol.twoff[0][0];
+  const cx<T>* tw_rev = f.twp + f.pcol.twoff[1][0];
+  constexpr int R0 = ct_radix(G.xp, false, 0);
+  constexpr int RP = ct_radix(G.xp, false, G.xp.np - 1);
+  constexpr int RS = ct_radix(G.xp, true, G.xp.np - 1);
+  const BufC32 ub(f.u0, (unsigned long long)3 * SST * 8);
+  const BufC32 ab(f.acc, (unsigned long long)3 * SST * 8);
+
+  // state offsets of this thread's modes (out of range outside the 2/3 mask)
+  unsigned soff[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    int e, t;
+    ct_elem<RP, L, NT, Lay>(i, tid, e, t);
+    const int kx = e < N / 2 ? e : e - N;
+    const int ax = kx < 0 ? -kx : kx;
+    const int kz = j0 + t;
+    soff[i] = (3 * ax < N && 3 * ay < N && kz < JM)
+                  ? (unsigned)(((long long)e * N + iy) * SP + kz)
+                  : 0xFFFFFFFFu;
+  }
+  auto so = [&](int i, int c) -> unsigned {
+    return soff[i] == 0xFFFFFFFFu ? kOob : (soff[i] + (unsigned)(c * SST)) * 8u;
+  };
+
+  cx<T> a0[E], a1[E], a2[E];
+  if constexpr (stage != kStagePre) {
+    constexpr int nf = ROT ? 3 : 5;
+#pragma unroll
+    for (int i = 0; i < E; ++i) a0[i] = a1[i] = a2[i] = czero<T>();
+    static_for<0, nf>([&](auto cc) {
+      constexpr int c = decltype(cc)::value;
+      cx<T> v[E];
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        int e, t;
+        ct_elem<R0, L, NT, Lay>(i, tid, e, t);
+        v[i] = f.W[c * FS + ((long long)e * N + iy) * LDW + j0 + t];
+      }
+      fft_ct<E, false, NT, Lay, G.xp>(v, lds, tw_fwd, tid);
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        if constexpr (ROT) {
+          if constexpr (c == 0) a0[i] = v[i];
+          else if constexpr (c == 1) a1[i] = v[i];
+          else a2[i] = v[i];
+        } else {
+          int e, t;
+          ct_elem<RP, L, NT, Lay>(i, tid, e, t);
+          const T kx = (T)(e < N / 2 ? e : e - N), fy = (T)ky, kz = (T)(j0 + t);
+          const cx<T> mt = mul_mi(v[i]);  // -i T_c;  N_i = -i k_j T_ij, T33 = -T11 - T22
+          if constexpr (c == 0) {
+            a0[i] = cadd(a0[i], cscl(kx, mt));
+            a2[i] = csub(a2[i], cscl(kz, mt));
+          } else if constexpr (c == 1) {
+            a1[i] = cadd(a1[i], cscl(fy, mt));
+            a2[i] = csub(a2[i], cscl(kz, mt));
+          } else if constexpr (c == 2) {
+            a0[i] = cadd(a0[i], cscl(fy, mt));
+            a1[i] = cadd(a1[i], cscl(kx, mt));
+          } else if constexpr (c == 3) {
+            a0[i] = cadd(a0[i], cscl(kz, mt));
+            a2[i] = cadd(a2[i], cscl(kx, mt));
+          } else {
+            a1[i] = cadd(a1[i], cscl(kz, mt));
+            a2[i] = cadd(a2[i], cscl(fy, mt));
+          }
+        }
+      }
+    });
+  }
+  {
+    const IfScalars s = load_if_scalars(f.prm, NH);
+    const T dt = (T)f.prm[2 * NH + PrmLayout::kDt];
+    const T scale = f.scale;
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int e, t;
+      ct_elem<RP, L, NT, Lay>(i, tid, e, t);
+      const int kx = e < N / 2 ? e : e - N;
+      const int ax = kx < 0 ? -kx : kx;
+      const int kz = j0 + t;
+      const cx<T> u0a = ub.load(so(i, 0)), u0b = ub.load(so(i, 1)), u0c = ub.load(so(i, 2));
+      cx<T> ua = u0a, ubv = u0b, uc = u0c;
+      if constexpr (stage != kStagePre) {
+        const int k2 = kx * kx + ky * ky + kz * kz;
+        const T fx = (T)kx, fy = (T)ky, fz = (T)kz;
+        const T ik2 = k2 > 0 ? __builtin_amdgcn_rcpf((T)k2) : (T)0;
+        const cx<T> d = cscl(ik2, cadd(cadd(cscl(fx, a0[i]), cscl(fy, a1[i])), cscl(fz, a2[i])));
+        const cx<T> n0 = csub(a0[i], cscl(fx, d));
+        const cx<T> n1 = csub(a1[i], cscl(fy, d));
+        const cx<T> n2 = csub(a2[i], cscl(fz, d));
+        double E1, Eh;
+        if_factors(f.prm, NH, s, ax, ay, kz, k2, E1, Eh);
+        cx<T> c0 = czero<T>(), c1 = czero<T>(), c2 = czero<T>();
+        if constexpr (stage >= 1) {
+          c0 = ab.load(so(i, 0));
+          c1 = ab.load(so(i, 1));
+          c2 = ab.load(so(i, 2));
+        }
+        ua = rk_value<stage, T>(dt, (T)E1, (T)Eh, n0, u0a, c0);
+        ubv = rk_value<stage, T>(dt, (T)E1, (T)Eh, n1, u0b, c1);
+        uc = rk_value<stage, T>(dt, (T)E1, (T)Eh, n2, u0c, c2);
+        if constexpr (stage < 3) {
+          ab.store(so(i, 0), c0);
+          ab.store(so(i, 1), c1);
+          ab.store(so(i, 2), c2);
+        } else {
+          ub.store(so(i, 0), ua);
+          ub.store(so(i, 1), ubv);
+          ub.store(so(i, 2), uc);
+        }
+      }
+      if (soff[i] == 0xFFFFFFFFu) {  // 2/3 mask: no nonlinear term outside it
+        ua = ubv = uc = czero<T>();
+      }
+      a0[i] = cscl(scale, ua);
+      a1[i] = cscl(scale, ubv);
+      a2[i] = cscl(scale, uc);
+    }
+  }
+  constexpr int nin = ROT ? 6 : 3;
+  static_for<0, nin>([&](auto cc) {
+    constexpr int c = decltype(cc)::value;
+    cx<T> v[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      if constexpr (c == 0) {
+        v[i] = a0[i];
+      } else if constexpr (c == 1) {
+        v[i] = a1[i];
+      } else if constexpr (c == 2) {
+        v[i] = a2[i];
+      } else {
+        int e, t;
+        ct_elem<RP, L, NT, Lay>(i, tid, e, t);
+        const T fx = (T)(e < N / 2 ? e : e - N), fy = (T)ky, fz = (T)(j0 + t);
+        // omega = i k x u
+        if constexpr (c == 3) v[i] = mul_i(csub(cscl(fy, a2[i]), cscl(fz, a1[i])));
+        else if constexpr (c == 4) v[i] = mul_i(csub(cscl(fz, a0[i]), cscl(fx, a2[i])));
+        else v[i] = mul_i(csub(cscl(fx, a1[i]), cscl(fy, a0[i])));
+      }
+    }
+    fft_ct<E, true, NT, Lay, G.xp>(v, lds, tw_rev, tid);
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      int e, t;
+      ct_elem<RS, L, NT, Lay>(i, tid, e, t);
+      f.W[c * FS + ((long long)e * N + iy) * LDW + j0 + t] = v[i];
+    }
+  });
+}
+
+// ------------------------------------------------------------------------------ launchers
+bool ct3d_supported(int n) {
+  switch (n) {
+#define PSX_X(NV) case NV:
+    PSX_CT3D_SIZES(PSX_X)
+#undef PSX_X
+    return true;
+    default:
+      return false;
+  }
+}
+
+template <int N, bool ROT>
+static hipError_t ct3_launch(const Flow<float>& f, int which, int arg, hipStream_t st) {
+  constexpr Geo3D G = G3<N>::g;
+  constexpr int NJT = G.ldw / G.T;
+  switch (which) {
+    case 0: {
+      auto fn = kc_line3d<N, ROT>;
+      static const bool attr_ = set_lds_max((const void*)fn);
+      (void)attr_;
+      size_t lds = (size_t)G.lineT * ct_rows_pitch(N) * 8;
+      const size_t red = (size_t)3 * G.lineNT * sizeof(double);
+      if (lds < red) lds = red;
+      hipLaunchKernelGGL(fn, dim3(N * N / G.lineT), dim3(G.lineNT), lds, st, f, arg);
+      break;
+    }
+    case 1:
+    case 3: {
+      const size_t lds = (size_t)N * G.T * 8;
+      if (which == 1)
+        hipLaunchKernelGGL((kc_ypass3d<N, ROT, false>), dim3(N * NJT), dim3(G.yNT), lds, st, f);
+      else
+        hipLaunchKernelGGL((kc_ypass3d<N, ROT, true>), dim3(N * NJT), dim3(G.yNT), lds, st, f);
+      break;
+    }
+    case 2: {
+      const size_t lds = (size_t)N * G.T * 8;
+      const dim3 grid(N * NJT), block(G.xNT);
+      switch (arg) {
+        case kStagePre: hipLaunchKernelGGL((kc_spec3d<N, ROT, kStagePre>), grid, block, lds, st, f); break;
+        case 0: hipLaunchKernelGGL((kc_spec3d<N, ROT, 0>), grid, block, lds, st, f); break;
+        case 1: hipLaunchKernelGGL((kc_spec3d<N, ROT, 1>), grid, block, lds, st, f); break;
+        case 2: hipLaunchKernelGGL((kc_spec3d<N, ROT, 2>), grid, block, lds, st, f); break;
+        case 3: hipLaunchKernelGGL((kc_spec3d<N, ROT, 3>), grid, block, lds, st, f); break;
+        default: return hipErrorInvalidValue;
+      }
+      break;
+    }
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// which: 0 z lines (arg = flags), 1 y forward, 2 x pass with physics (arg = stage),
+// 3 y inverse.
+hipError_t launch_ct3d(const Flow<float>& f, int which, int arg, hipStream_t st) {
+  switch (f.n) {
+#define PSX_X(NV)                                                            \
+  case NV:                                                                   \
+    return f.rot ? ct3_launch<NV, true>(f, which, arg, st)                   \
+                 : ct3_launch<NV, false>(f, which, arg, st);
+    PSX_CT3D_SIZES(PSX_X)
+#undef PSX_X
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace psx

@@ -54,6 +54,11 @@ hipError_t launch_spec3d(const Flow<T>& f, const Launch& lc, int stage, hipStrea
 bool ct2d_supported(int n);
 hipError_t launch_ct2d(const Flow<float>& f, int which, int arg, hipStream_t st);
 
+// Compile-time specialised 3D fp32 kernels (psx_k3d_ct.hip).  which: 0 z lines (arg = flags),
+// 1 y forward, 2 x pass with physics (arg = stage), 3 y inverse.
+bool ct3d_supported(int n);
+hipError_t launch_ct3d(const Flow<float>& f, int which, int arg, hipStream_t st);
+
 // Utility / diagnostics kernels (psx_kutil.hip).
 hipError_t launch_params(double* prm, int nh, double nu, double dt, double alpha, double amp,
                          double klo2, double khi2, double ka2, hipStream_t st);

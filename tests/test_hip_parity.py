@@ -95,6 +95,10 @@ def test_2d_sizes_and_four_step(n, dtype, steps):
 
 @pytest.mark.parametrize("n,form,dtype", [(48, "basdevant", "float32"),
                                           (64, "rotational", "float32"),
+                                          (64, "basdevant", "float32"),
+                                          (96, "basdevant", "float32"),
+                                          (96, "rotational", "float32"),
+                                          (128, "basdevant", "float32"),
                                           (48, "rotational", "float64"),
                                           (32, "basdevant", "float64")])
 def test_3d_sizes(n, form, dtype):
@@ -113,7 +117,8 @@ def test_3d_sizes(n, form, dtype):
 
 
 @pytest.mark.parametrize("dim,n,dtype", [(2, 32, "float64"), (3, 16, "float64"),
-                                         (2, 64, "float32"), (3, 24, "float32")])
+                                         (2, 64, "float32"), (3, 24, "float32"),
+                                         (3, 64, "float32")])
 def test_device_diagnostics_match_oracle(dim, n, dtype):
     s = _make("hip/fused", Config(n=n, dim=dim, dtype=dtype))
     st = _ic(n, dim, seed=7)
