@@ -122,6 +122,51 @@ int psx_reduce(psx_ctx* c, const void* state, int pitch, double klo2, double khi
 int psx_spectrum(psx_ctx* c, const void* state, int pitch, double* out, int nshell,
                  void* stream);
 
+/* ---- device-resident time loop ------------------------------------------------------
+ * The loop of src/pyspectral/simulate.py:66-96 (run) with its controller on the device:
+ * every cfl_every steps dt = min(dt_max, cfl * (2 pi / n) / sum_i max|u_i|) (:31-37, the
+ * maxima taken by the first stage of the same step) and, when power >= 0, the band growth
+ * rate A = power / (2 E_band) (:80-83); each step h = min(dt, t_end - t, next_diag - t);
+ * a record (t, E, Z) after every step that reaches a diagnostics time and at t_end (:86-96).
+ * Steps run in chunks (one captured graph of nsteps controller steps); chunks enqueued past
+ * the end are no-ops, so the host can keep several in flight and poll rarely. */
+typedef struct psx_run_params {
+  double t0;        /* current time */
+  double t_end;     /* end time (may be +inf with kmax > 0) */
+  double dt_fixed;  /* > 0: fixed dt; otherwise CFL-adaptive with cfl and dt_max */
+  double cfl, dt_max;
+  double diag_dt;   /* > 0: a record every diag_dt (steps land on it); 0: records at the ends */
+  double power;     /* >= 0: constant-power band forcing target P; < 0: A stays amp0 */
+  double amp0;      /* band growth rate in force at the start */
+  long long kmax;   /* > 0: stop after kmax steps */
+  int cfl_every;    /* >= 1 */
+  int cap_rec;      /* capacity of the record ring (t, E, Z) */
+  int cap_step;     /* capacity of the step ring (h, A) */
+  int record_start; /* 1: record the initial state */
+  double next_diag; /* first record time when diag_dt > 0 (NaN: t0 + diag_dt) */
+} psx_run_params;
+
+typedef struct psx_run_state {
+  double t, amp, cur_dt, next_diag;
+  long long k;      /* steps taken; step i is in ring slot i % cap_step */
+  long long nrec;   /* records taken; record i is in ring slot i % cap_rec */
+  int done;
+} psx_run_state;
+
+/* Start a run from the prepared state (after psx_prepare); synchronises the stream. */
+int psx_run_begin(psx_ctx* c, const psx_run_params* rp, void* stream);
+/* Enqueue nsteps controller steps (as one graph if use_graph, captured on first use),
+ * followed by a copy of the controller state to a pinned host slot (8 slots). */
+int psx_run_chunk(psx_ctx* c, int nsteps, int use_graph, void* stream);
+/* Wait for the chunk enqueued `lag` chunks before the latest (0 <= lag < 8) and return the
+ * controller state after it, without synchronising the stream. */
+int psx_run_peek(psx_ctx* c, int lag, psx_run_state* out);
+/* Synchronise; read the controller state and, when non-null, the rings (3 cap_rec doubles
+ * into rec, 2 cap_step doubles into steps). */
+int psx_run_poll(psx_ctx* c, psx_run_state* out, double* rec, double* steps, void* stream);
+/* End a run: the context steps eagerly again (psx_set_step must follow before psx_step). */
+int psx_run_end(psx_ctx* c, void* stream);
+
 /* Message of the last failure on this thread. */
 const char* psx_last_error(void);
 

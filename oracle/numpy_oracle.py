@@ -373,10 +373,12 @@ def run(solver: Any, t_end: float, diag: Any, *, dt: float | None = None, cfl: f
         dt_max: float = 0.05, cfl_every: int = 10, diag_dt: float | None = None,
         target_power: float | None = None, klo: float = 0.0, khi: float = 0.0,
         callback: Callable[[Any], None] | None = None, callback_dt: float | None = None,
-        dt_sequence: list[float] | None = None) -> RunRecord:
+        dt_sequence: list[float] | None = None,
+        amp_sequence: list[float] | None = None) -> RunRecord:
     """simulate.py:40-96.  ``diag`` provides energy/enstrophy/band_energy/max_velocity_sum on
     the solver's state.  ``dt_sequence`` replays a recorded dt sequence instead of the CFL
-    rule (used to give the oracle exactly the steps another solver took)."""
+    rule (used to give the oracle exactly the steps another solver took); ``amp_sequence``
+    likewise replays the band growth rate of every step."""
     res = RunRecord()
 
     def record() -> None:
@@ -390,6 +392,8 @@ def run(solver: Any, t_end: float, diag: Any, *, dt: float | None = None, cfl: f
     next_cb = solver.t + callback_dt if callback_dt else math.inf
     if dt_sequence is not None:
         seq = iter(dt_sequence)
+    if amp_sequence is not None:
+        aseq = iter(amp_sequence)
     cur_dt = dt if dt is not None else cfl_dt(diag.max_velocity_sum(solver.get_state()),
                                               solver.n, cfl, dt_max)
     t0 = time.perf_counter()
@@ -406,6 +410,8 @@ def run(solver: Any, t_end: float, diag: Any, *, dt: float | None = None, cfl: f
         h = min(cur_dt, t_end - solver.t, next_diag - solver.t, next_cb - solver.t)
         if dt_sequence is not None:
             h = next(seq)
+        if amp_sequence is not None:
+            solver.set_forcing_amplitude(next(aseq))
         solver.step(h)
         res.dt.append(h)
         res.amp.append(solver.force_amp)

@@ -27,6 +27,20 @@ class Layout(ctypes.Structure):
                 ("state_pitch", c_int)]
 
 
+class RunParams(ctypes.Structure):
+    _fields_ = [("t0", c_double), ("t_end", c_double), ("dt_fixed", c_double), ("cfl", c_double),
+                ("dt_max", c_double), ("diag_dt", c_double), ("power", c_double),
+                ("amp0", c_double), ("kmax", c_longlong), ("cfl_every", c_int),
+                ("cap_rec", c_int), ("cap_step", c_int), ("record_start", c_int),
+                ("next_diag", c_double)]
+
+
+class RunState(ctypes.Structure):
+    _fields_ = [("t", c_double), ("amp", c_double), ("cur_dt", c_double),
+                ("next_diag", c_double), ("k", c_longlong), ("nrec", c_longlong),
+                ("done", c_int)]
+
+
 # name -> (restype, argtypes); the exported C ABI
 SIGNATURES = {
     "psx_query_layout": (c_int, [POINTER(FlowParams), POINTER(Layout)]),
@@ -43,6 +57,11 @@ SIGNATURES = {
     "psx_spectrum": (c_int, [c_void_p, c_void_p, c_int, POINTER(c_double), c_int, c_void_p]),
     "psx_timing_begin": (c_int, [c_void_p]),
     "psx_timing_end": (c_int, [c_void_p, c_void_p, POINTER(c_double), POINTER(c_int)]),
+    "psx_run_begin": (c_int, [c_void_p, POINTER(RunParams), c_void_p]),
+    "psx_run_chunk": (c_int, [c_void_p, c_int, c_int, c_void_p]),
+    "psx_run_peek": (c_int, [c_void_p, c_int, POINTER(RunState)]),
+    "psx_run_poll": (c_int, [c_void_p, POINTER(RunState), c_void_p, c_void_p, c_void_p]),
+    "psx_run_end": (c_int, [c_void_p, c_void_p]),
     "psx_last_error": (c_char_p, []),
     "psx_device_info": (c_int, [c_char_p, c_int]),
 }

@@ -9,6 +9,7 @@ enqueued on torch's current stream.
 from __future__ import annotations
 
 import ctypes
+import math
 from typing import Any
 
 import numpy as np
@@ -156,6 +157,78 @@ class HipSolver(SpectralSolver):
             _lib.check(self.lib.psx_step_graph(self.ctx, st))
         else:
             _lib.check(self.lib.psx_step(self.ctx, st))
+
+    # ------------------------------------------------------------------ device time loop
+    def run_device(self, t_end: float, *, dt: float | None = None, cfl: float = 0.5,
+                   dt_max: float = 0.05, cfl_every: int = 10, diag_dt: float | None = None,
+                   target_power: float | None = None, kmax: int = 0, chunk: int | None = None,
+                   use_graph: bool = True, depth: int = 3, record_start: bool = True,
+                   first_diag: float | None = None) -> dict[str, Any]:
+        """Integrate to ``t_end`` (or ``kmax`` steps) with the controller on the device
+        (src/pyspectral/simulate.py:40-96 semantics, no callback).  Chunks of ``chunk`` steps
+        are enqueued ``depth`` deep; the host waits only on the oldest chunk's state copy.
+        Returns the records (t, E, Z) and the step history (h, A)."""
+        lib, st = self.lib, _lib.stream_ptr()
+        chunk = int(chunk or cfl_every)
+        cap_step = 1 << 14
+        cap_rec = 1 << 12
+        if chunk * (depth + 1) > cap_step // 2:
+            raise ValueError("chunk * (depth + 1) exceeds the step ring")
+        rp = _lib.RunParams(t0=float(self.t), t_end=float(t_end),
+                            dt_fixed=float(dt) if dt is not None else 0.0, cfl=float(cfl),
+                            dt_max=float(dt_max), diag_dt=float(diag_dt or 0.0),
+                            power=-1.0 if target_power is None else float(target_power),
+                            amp0=float(self.force_amp), kmax=int(kmax),
+                            cfl_every=int(cfl_every), cap_rec=cap_rec, cap_step=cap_step,
+                            record_start=int(record_start),
+                            next_diag=float(first_diag) if first_diag is not None else math.nan)
+        _lib.check(lib.psx_run_begin(self.ctx, ctypes.byref(rp), st))
+        self._key = None  # the controller rewrites the parameter block
+        rec = np.zeros(3 * cap_rec)
+        steps = np.zeros(2 * cap_step)
+        out = {"t": [], "energy": [], "enstrophy": [], "dt": [], "amp": []}
+        seen_k = seen_rec = 0
+        state = _lib.RunState()
+
+        def drain() -> None:
+            nonlocal seen_k, seen_rec
+            _lib.check(lib.psx_run_poll(self.ctx, ctypes.byref(state),
+                                        rec.ctypes.data_as(ctypes.c_void_p),
+                                        steps.ctypes.data_as(ctypes.c_void_p), st))
+            for i in range(seen_rec, state.nrec):
+                j = 3 * (i % cap_rec)
+                out["t"].append(float(rec[j]))
+                out["energy"].append(float(rec[j + 1]))
+                out["enstrophy"].append(float(rec[j + 2]))
+            for i in range(seen_k, state.k):
+                j = 2 * (i % cap_step)
+                out["dt"].append(float(steps[j]))
+                out["amp"].append(float(steps[j + 1]))
+            seen_k, seen_rec = state.k, state.nrec
+
+        try:
+            queued = 0
+            peek = _lib.RunState()
+            while True:
+                _lib.check(lib.psx_run_chunk(self.ctx, chunk, int(use_graph), st))
+                queued += 1
+                if queued < depth:
+                    continue
+                _lib.check(lib.psx_run_peek(self.ctx, depth - 1, ctypes.byref(peek)))
+                if peek.done:
+                    break
+                if (peek.k - seen_k) + chunk * depth > cap_step // 2 or \
+                        (peek.nrec - seen_rec) + chunk * depth > cap_rec // 2:
+                    drain()
+            drain()
+        finally:
+            _lib.check(lib.psx_run_end(self.ctx, st))
+        self.nstep += state.k
+        self.t = float(state.t)
+        self.force_amp = float(state.amp)
+        out["steps"] = int(state.k)
+        out["finished"] = bool(state.t >= t_end - 1e-12 * max(1.0, t_end))
+        return out
 
     def _make_diag(self) -> Any:
         from pyspectral_amd.diagnostics import DeviceDiagnostics

@@ -154,6 +154,20 @@ struct psx_ctx {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int nshell_max = 0;
+  // device controller (psx_run_*): one member, its state, rings and chunk graphs
+  DrvMember* dmem = nullptr;
+  DrvState* dstate = nullptr;
+  double* dring = nullptr;       // records (3 cap_rec) then steps (2 cap_step)
+  double* dpartial = nullptr;
+  DrvMember hmem{};
+  int run_chunk = 0, run_mode = 0;
+  // controller state copied to pinned host slots after each chunk (psx_run_peek)
+  static constexpr int kSlots = 8;
+  DrvState* hslots = nullptr;
+  hipEvent_t slot_ev[kSlots] = {};
+  long long nchunk = 0;
+  hipGraph_t rgraph = nullptr;
+  hipGraphExec_t rgexec = nullptr;
   bool ct = false;  // compile-time specialised fp32 kernels for this size (2D or 3D)
   // per-launch event timing of eager steps (psx_timing_*)
   bool timing = false;
@@ -410,25 +424,35 @@ int enqueue_prepare(psx_ctx* c, hipStream_t st) {
   return 0;
 }
 
+// One IF-RK4 step.  drv: 0 plain step (parameters from psx_set_step); 1 controller step
+// (device controller around the stages, psx_drv.h); 2 controller step with CFL maxima taken
+// by the first stage's line kernel.
 template <typename T>
-int enqueue_step(psx_ctx* c, hipStream_t st) {
+int enqueue_step(psx_ctx* c, hipStream_t st, int drv = 0) {
   Flow<T>& f = flow<T>(c);
   for (int s = 0; s < 4; ++s) {
-    bool done = false;
+    const int lflags = (s == 0 && drv == 2) ? kLineWantMax : 0;
+    bool ct = false;
+    if constexpr (sizeof(T) == 4) ct = c->ct;
+    // line kernel (real axis): inverse transform, products, forward transform
+    PSX_MARK(PSX_KIND_LINE);
     if constexpr (sizeof(T) == 4) {
-      if (c->ct && f.dim == 3) {
-        PSX_MARK(PSX_KIND_LINE);
-        PSX_HIP(launch_ct3d(f, 0, 0, st));
+      if (ct) PSX_HIP(f.dim == 3 ? launch_ct3d(f, 0, lflags, st) : launch_ct2d(f, 0, lflags, st));
+    }
+    if (!ct) {
+      if (f.dim == 2) PSX_HIP(launch_line2d<T>(f, c->line, lflags, st));
+      else PSX_HIP(launch_line3d<T>(f, c->line, lflags, st));
+    }
+    if (s == 0 && drv) PSX_HIP(launch_drv(0, c->dmem, 1, st));
+    if constexpr (sizeof(T) == 4) {
+      if (ct && f.dim == 3) {
         PSX_MARK(PSX_KIND_MID_FWD);
         PSX_HIP(launch_ct3d(f, 1, 0, st));
         PSX_MARK(PSX_KIND_SPEC);
         PSX_HIP(launch_ct3d(f, 2, s, st));
         PSX_MARK(PSX_KIND_MID_INV);
         PSX_HIP(launch_ct3d(f, 3, 0, st));
-        done = true;
-      } else if (c->ct) {
-        PSX_MARK(PSX_KIND_LINE);
-        PSX_HIP(launch_ct2d(f, 0, 0, st));
+      } else if (ct) {
         if (f.n1 > 1) {
           PSX_MARK(PSX_KIND_OUTER_FWD);
           PSX_HIP(launch_ct2d(f, 2, 0, st));
@@ -439,13 +463,10 @@ int enqueue_step(psx_ctx* c, hipStream_t st) {
           PSX_MARK(PSX_KIND_OUTER_INV);
           PSX_HIP(launch_ct2d(f, 3, 0, st));
         }
-        done = true;
       }
     }
-    if (done) {
+    if (ct) {
     } else if (f.dim == 2) {
-      PSX_MARK(PSX_KIND_LINE);
-      PSX_HIP(launch_line2d<T>(f, c->line, 0, st));
       if (f.n1 > 1) {
         PSX_MARK(PSX_KIND_OUTER_FWD);
         PSX_HIP(launch_outer2d<T>(f, c->outerF, false, st));
@@ -458,8 +479,6 @@ int enqueue_step(psx_ctx* c, hipStream_t st) {
         PSX_HIP(launch_spec2d<T>(f, c->spec, false, s, st));
       }
     } else {
-      PSX_MARK(PSX_KIND_LINE);
-      PSX_HIP(launch_line3d<T>(f, c->line, 0, st));
       PSX_MARK(PSX_KIND_MID_FWD);
       PSX_HIP(launch_ypass3d<T>(f, c->yF, false, f.rot ? 3 : 5, st));
       PSX_MARK(PSX_KIND_SPEC);
@@ -467,6 +486,10 @@ int enqueue_step(psx_ctx* c, hipStream_t st) {
       PSX_MARK(PSX_KIND_MID_INV);
       PSX_HIP(launch_ypass3d<T>(f, c->yI, true, f.rot ? 6 : 3, st));
     }
+  }
+  if (drv) {
+    PSX_HIP(launch_drv(1, c->dmem, 1, st));
+    PSX_HIP(launch_drv(2, c->dmem, 1, st));
   }
   PSX_MARK(-1);
   return 0;
@@ -603,6 +626,15 @@ int psx_destroy(psx_ctx* c) {
   if (c->stats) (void)hipFree(c->stats);
   if (c->partial) (void)hipFree(c->partial);
   if (c->dout) (void)hipFree(c->dout);
+  if (c->rgexec) (void)hipGraphExecDestroy(c->rgexec);
+  if (c->rgraph) (void)hipGraphDestroy(c->rgraph);
+  if (c->dmem) (void)hipFree(c->dmem);
+  if (c->dstate) (void)hipFree(c->dstate);
+  if (c->dring) (void)hipFree(c->dring);
+  if (c->dpartial) (void)hipFree(c->dpartial);
+  if (c->hslots) (void)hipHostFree(c->hslots);
+  for (hipEvent_t e : c->slot_ev)
+    if (e) (void)hipEventDestroy(e);
   delete c;
   return 0;
 }
@@ -722,6 +754,192 @@ int psx_spectrum(psx_ctx* c, const void* state, int pitch, double* out, int nshe
                                    c->dout, nshell, st));
   PSX_HIP(hipMemcpyAsync(out, c->dout, nshell * sizeof(double), hipMemcpyDeviceToHost, st));
   PSX_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+// ------------------------------------------------------------------ device controller
+int psx_run_begin(psx_ctx* c, const psx_run_params* rp, void* stream) {
+  if (!c || !rp) return fail("psx_run_begin: null argument");
+  if (rp->cfl_every < 1) return fail("psx_run_begin: cfl_every must be >= 1");
+  if (rp->cap_rec < 1 || rp->cap_step < 1) return fail("psx_run_begin: ring capacities < 1");
+  if (!(rp->dt_fixed > 0.0) && !(rp->cfl > 0.0 && rp->dt_max > 0.0))
+    return fail("psx_run_begin: need dt_fixed > 0, or cfl > 0 and dt_max > 0");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (c->hmem.cap_rec != rp->cap_rec || c->hmem.cap_step != rp->cap_step) {
+    if (c->dring) (void)hipFree(c->dring);
+    c->dring = nullptr;
+    PSX_HIP(hipMalloc(&c->dring, (size_t)(3 * rp->cap_rec + 2 * rp->cap_step) * sizeof(double)));
+  }
+  if (!c->dmem) {
+    PSX_HIP(hipMalloc(&c->dmem, sizeof(DrvMember)));
+    PSX_HIP(hipMalloc(&c->dstate, sizeof(DrvState)));
+    PSX_HIP(hipMalloc(&c->dpartial, 2 * kDrvRedBlocks * sizeof(double)));
+    PSX_HIP(hipHostMalloc(&c->hslots, psx_ctx::kSlots * sizeof(DrvState)));
+    for (hipEvent_t& e : c->slot_ev) PSX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  c->nchunk = 0;
+  DrvMember m{};
+  m.u0 = c->fp64 ? (const void*)c->f64.u0 : (const void*)c->f32.u0;
+  m.prm = c->prm;
+  m.stats = c->stats;
+  m.s = c->dstate;
+  m.rec = c->dring;
+  m.steps = c->dring + 3 * rp->cap_rec;
+  m.partial = c->dpartial;
+  m.nu = c->p.nu;
+  m.alpha = c->p.alpha;
+  m.klo2 = c->p.klo2;
+  m.khi2 = c->p.khi2;
+  m.ka2 = c->p.ka2;
+  m.n = c->p.n;
+  m.dim = c->p.dim;
+  m.sp = c->fp64 ? c->f64.sp : c->f32.sp;
+  m.fp64 = c->fp64;
+  m.cap_rec = rp->cap_rec;
+  m.cap_step = rp->cap_step;
+  DrvState s{};
+  s.t = rp->t0;
+  s.t_end = rp->t_end;
+  s.eps = 1e-12 * std::fmax(1.0, std::isfinite(rp->t_end) ? rp->t_end : 1.0);
+  s.dt_fixed = rp->dt_fixed > 0.0 ? rp->dt_fixed : 0.0;
+  s.cfl = rp->cfl;
+  s.dt_max = rp->dt_max;
+  s.dx = 2.0 * M_PI / c->p.n;
+  s.diag_dt = rp->diag_dt > 0.0 ? rp->diag_dt : 0.0;
+  s.next_diag = !(rp->diag_dt > 0.0)          ? INFINITY
+                : std::isfinite(rp->next_diag) ? rp->next_diag
+                                               : rp->t0 + rp->diag_dt;
+  s.power = rp->power;
+  s.amp = rp->amp0;
+  s.cur_dt = s.dt_fixed;
+  s.h = 0.0;
+  s.last_rec_t = -INFINITY;
+  s.k = 0;
+  s.kmax = rp->kmax > 0 ? rp->kmax : (long long)9e18;
+  s.cfl_every = rp->cfl_every;
+  s.nrec = 0;
+  s.done = !(s.t < s.t_end - s.eps) || s.kmax <= 0;
+  s.active = !s.done;
+  s.record = rp->record_start ? 1 : 0;
+  c->hmem = m;
+  c->run_mode = s.dt_fixed > 0.0 ? 1 : 2;
+  PSX_HIP(hipMemcpyAsync(c->dmem, &c->hmem, sizeof(DrvMember), hipMemcpyHostToDevice, st));
+  PSX_HIP(hipMemcpyAsync(c->dstate, &s, sizeof(DrvState), hipMemcpyHostToDevice, st));
+  PSX_HIP(launch_fill_u32(reinterpret_cast<unsigned*>(c->stats), 0u, 6, st));
+  if (s.active) {
+    // the parameter block marks the flow active; the controller rewrites it every step
+    const int nh = c->p.n / 2 + 1;
+    PSX_HIP(launch_params(c->prm, nh, c->p.nu, 0.0, c->p.alpha, rp->amp0, c->p.klo2,
+                          c->p.khi2, c->p.ka2, st));
+  } else {
+    PSX_HIP(launch_drv(4, c->dmem, 1, st));
+  }
+  PSX_HIP(launch_drv(2, c->dmem, 1, st));  // initial record when asked
+  // the host copies may be read by the kernels only after they land
+  PSX_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+int psx_run_chunk(psx_ctx* c, int nsteps, int use_graph, void* stream) {
+  if (!c) return fail("psx_run_chunk: null context");
+  if (!c->dmem || !c->run_mode) return fail("psx_run_chunk: no run begun");
+  if (nsteps < 1) return fail("psx_run_chunk: nsteps must be >= 1");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const int drv = c->run_mode;
+  // a slot is reused only after the host no longer waits on it
+  const int slot = (int)(c->nchunk % psx_ctx::kSlots);
+  PSX_HIP(hipEventSynchronize(c->slot_ev[slot]));
+  if (!use_graph) {
+    for (int k = 0; k < nsteps; ++k) {
+      int r = c->fp64 ? enqueue_step<double>(c, st, drv) : enqueue_step<float>(c, st, drv);
+      if (r) return r;
+    }
+    PSX_HIP(hipMemcpyAsync(c->hslots + slot, c->dstate, sizeof(DrvState),
+                           hipMemcpyDeviceToHost, st));
+    PSX_HIP(hipEventRecord(c->slot_ev[slot], st));
+    ++c->nchunk;
+    return 0;
+  }
+  const int key = nsteps * 4 + drv;
+  if (!c->rgexec || c->run_chunk != key) {
+    if (c->rgexec) (void)hipGraphExecDestroy(c->rgexec);
+    if (c->rgraph) (void)hipGraphDestroy(c->rgraph);
+    c->rgexec = nullptr;
+    c->rgraph = nullptr;
+    if (!c->cap) PSX_HIP(hipStreamCreateWithFlags(&c->cap, hipStreamNonBlocking));
+    const bool was_timing = c->timing;
+    c->timing = false;
+    PSX_HIP(hipStreamBeginCapture(c->cap, hipStreamCaptureModeRelaxed));
+    int r = 0;
+    for (int k = 0; k < nsteps && !r; ++k)
+      r = c->fp64 ? enqueue_step<double>(c, c->cap, drv) : enqueue_step<float>(c, c->cap, drv);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->cap, &g);
+    c->timing = was_timing;
+    if (r) {
+      if (g) (void)hipGraphDestroy(g);
+      return r;
+    }
+    if (e != hipSuccess) return fail("psx_run_chunk: capture failed: %s", hipGetErrorString(e));
+    c->rgraph = g;
+    PSX_HIP(hipGraphInstantiate(&c->rgexec, c->rgraph, nullptr, nullptr, 0));
+    c->run_chunk = key;
+  }
+  PSX_HIP(hipGraphLaunch(c->rgexec, st));
+  PSX_HIP(hipMemcpyAsync(c->hslots + slot, c->dstate, sizeof(DrvState), hipMemcpyDeviceToHost,
+                         st));
+  PSX_HIP(hipEventRecord(c->slot_ev[slot], st));
+  ++c->nchunk;
+  return 0;
+}
+
+int psx_run_peek(psx_ctx* c, int lag, psx_run_state* out) {
+  if (!c || !out) return fail("psx_run_peek: null argument");
+  if (lag < 0 || lag >= psx_ctx::kSlots || lag >= c->nchunk)
+    return fail("psx_run_peek: lag %d outside [0, min(%d, %lld))", lag, psx_ctx::kSlots,
+                c->nchunk);
+  const int slot = (int)((c->nchunk - 1 - lag) % psx_ctx::kSlots);
+  PSX_HIP(hipEventSynchronize(c->slot_ev[slot]));
+  const DrvState& s = c->hslots[slot];
+  out->t = s.t;
+  out->amp = s.amp;
+  out->cur_dt = s.cur_dt;
+  out->next_diag = s.next_diag;
+  out->k = s.k;
+  out->nrec = s.nrec;
+  out->done = s.done;
+  return 0;
+}
+
+int psx_run_poll(psx_ctx* c, psx_run_state* out, double* rec, double* steps, void* stream) {
+  if (!c || !out) return fail("psx_run_poll: null argument");
+  if (!c->dmem) return fail("psx_run_poll: no run begun");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  DrvState s{};
+  PSX_HIP(hipMemcpyAsync(&s, c->dstate, sizeof s, hipMemcpyDeviceToHost, st));
+  if (rec)
+    PSX_HIP(hipMemcpyAsync(rec, c->hmem.rec, 3 * (size_t)c->hmem.cap_rec * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+  if (steps)
+    PSX_HIP(hipMemcpyAsync(steps, c->hmem.steps, 2 * (size_t)c->hmem.cap_step * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+  PSX_HIP(hipStreamSynchronize(st));
+  out->t = s.t;
+  out->amp = s.amp;
+  out->cur_dt = s.cur_dt;
+  out->next_diag = s.next_diag;
+  out->k = s.k;
+  out->nrec = s.nrec;
+  out->done = s.done;
+  return 0;
+}
+
+int psx_run_end(psx_ctx* c, void* stream) {
+  if (!c) return fail("psx_run_end: null context");
+  if (!c->dmem) return 0;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  PSX_HIP(launch_drv(3, c->dmem, 1, st));
+  c->run_mode = 0;
   return 0;
 }
 

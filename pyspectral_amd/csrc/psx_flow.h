@@ -22,7 +22,7 @@ namespace psx {
 // FP64 parameter block: [0, nh) exp(-nu m^2 dt), [nh, 2 nh) exp(-nu m^2 dt/2), then scalars.
 struct PrmLayout {
   static constexpr int kDt = 0, kEa1 = 1, kEah = 2, kEb1 = 3, kEbh = 4, kKlo2 = 5, kKhi2 = 6,
-                       kKa2 = 7, kBand = 8, kNScalars = 16;
+                       kKa2 = 7, kBand = 8, kActive = 9, kNScalars = 16;
 };
 
 template <typename T>

@@ -20,6 +20,7 @@ namespace psx {
 // ------------------------------------------------------------------------------ line kernel
 template <typename T, int E>
 __global__ void __launch_bounds__(512, 1) k_line2d(Flow<T> f, int flags) {
+  if (flow_inactive(f.prm, f.nh)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int tid = threadIdx.x, NT = blockDim.x;
@@ -112,6 +113,7 @@ __global__ void __launch_bounds__(512, 1) k_line2d(Flow<T> f, int flags) {
 // outer index k1 = g, the modes k = k1 + n1 k2 after the forward transform.
 template <typename T, int E, bool FOUR>
 __global__ void __launch_bounds__(512, 1) k_spec2d(Flow<T> f, int stage) {
+  if (flow_inactive(f.prm, f.nh)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int tid = threadIdx.x, NT = blockDim.x;
@@ -206,6 +208,7 @@ __global__ void __launch_bounds__(512, 1) k_spec2d(Flow<T> f, int stage) {
 // twiddle was applied by the inner kernel), store at row n2 x1 + x2.  Both work fields.
 template <typename T, int E, bool INV>
 __global__ void __launch_bounds__(512, 1) k_outer2d(Flow<T> f) {
+  if (flow_inactive(f.prm, f.nh)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int tid = threadIdx.x, NT = blockDim.x;

@@ -25,6 +25,7 @@ struct G2 {
 // --------------------------------------------------------------------------- line kernel
 template <int N>
 __global__ void __launch_bounds__(G2<N>::g.lineNT, 1) kc_line2d(Flow<float> f, int flags) {
+  if (flow_inactive(f.prm, f.nh)) return;
   using T = float;
   constexpr Geo2D G = G2<N>::g;
   constexpr int E = G.lineE, NT = G.lineNT, RT = G.lineT, L = N, JM = G.jm, LDW = G.ldw;
@@ -58,13 +59,14 @@ __global__ void __launch_bounds__(G2<N>::g.lineNT, 1) kc_line2d(Flow<float> f, i
   fft_ct<E, true, NT, Lay, G.line>(v, lds, tw_rev, tid);
 
   if (flags & kLineWantMax) {
-    double m[2] = {0.0, 0.0};
+    float m[2] = {0.f, 0.f};  // |u_i| maxima (exact in fp32)
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      m[0] = fmax(m[0], abs_or_inf((double)v[i].x));
-      m[1] = fmax(m[1], abs_or_inf((double)v[i].y));
+      m[0] = fmaxf(m[0], abs_or_inf(v[i].x));
+      m[1] = fmaxf(m[1], abs_or_inf(v[i].y));
     }
-    block_max_to<2>(m, f.stats, reinterpret_cast<double*>(smem_raw));
+    double md[2] = {(double)m[0], (double)m[1]};
+      block_max_to<2>(md, f.stats, reinterpret_cast<double*>(smem_raw));
   }
   if (flags & kLineMaxOnly) return;
 
@@ -107,6 +109,7 @@ __device__ __forceinline__ void z_pack(int j, cx<float> u, cx<float> v, cx<float
 // ----------------------------------------------------------------------- spectral kernel
 template <int N, int stage>
 __global__ void __launch_bounds__(G2<N>::g.specNT, 2) kc_spec2d(Flow<float> f) {
+  if (flow_inactive(f.prm, f.nh)) return;
   using T = float;
   constexpr Geo2D G = G2<N>::g;
   constexpr bool FOUR = G.n1 > 1;
@@ -229,6 +232,7 @@ __global__ void __launch_bounds__(G2<N>::g.specNT, 2) kc_spec2d(Flow<float> f) {
 template <int N, bool INV>
 __global__ void __launch_bounds__(G2<N>::g.outNT > 0 ? G2<N>::g.outNT : 64, 1)
     kc_outer2d(Flow<float> f) {
+  if (flow_inactive(f.prm, f.nh)) return;
   using T = float;
   constexpr Geo2D G = G2<N>::g;
   if constexpr (G.n1 > 1) {

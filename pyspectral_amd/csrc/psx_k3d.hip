@@ -108,6 +108,7 @@ __device__ __forceinline__ void line_store_real(const cx<T> (&v)[E], const Flow<
 
 template <typename T, int E>
 __global__ void __launch_bounds__(512, 1) k_line3d(Flow<T> f, int flags) {
+  if (flow_inactive(f.prm, f.nh)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int tid = threadIdx.x, NT = blockDim.x;
@@ -194,6 +195,7 @@ __global__ void __launch_bounds__(512, 1) k_line3d(Flow<T> f, int flags) {
 // Middle axis (y) of nfld fields: for fixed x, column tile j0..j0+yT, rows x n + y.
 template <typename T, int E, bool INV>
 __global__ void __launch_bounds__(512, 1) k_ypass3d(Flow<T> f, int nfld) {
+  if (flow_inactive(f.prm, f.nh)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int tid = threadIdx.x, NT = blockDim.x;
@@ -232,6 +234,7 @@ __global__ void __launch_bounds__(512, 1) k_ypass3d(Flow<T> f, int nfld) {
 // j0..j0+colT, rows x n + ky_idx.
 template <typename T, int E>
 __global__ void __launch_bounds__(512, 1) k_spec3d(Flow<T> f, int stage) {
+  if (flow_inactive(f.prm, f.nh)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int tid = threadIdx.x, NT = blockDim.x;

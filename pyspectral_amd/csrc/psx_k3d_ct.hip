@@ -47,6 +47,7 @@ __device__ __forceinline__ void z_pack3(int j, cx<float> u, cx<float> v, cx<floa
 // ----------------------------------------------------------------------------- z lines
 template <int N, bool ROT>
 __global__ void __launch_bounds__(G3<N>::g.lineNT, 1) kc_line3d(Flow<float> f, int flags) {
+  if (flow_inactive(f.prm, f.nh)) return;
   using T = float;
   constexpr Geo3D G = G3<N>::g;
   constexpr int E = G.lineE, NT = G.lineNT, RT = G.lineT, L = N, JM = G.jm, LDW = G.ldw;
@@ -116,14 +117,15 @@ __global__ void __launch_bounds__(G3<N>::g.lineNT, 1) kc_line3d(Flow<float> f, i
     load_single(z2, 2);
     fft_ct<E, true, NT, Lay, G.line>(z2, lds, tw_rev, tid);
     if (flags & kLineWantMax) {
-      double m[3] = {0.0, 0.0, 0.0};
+      float m[3] = {0.f, 0.f, 0.f};  // |u_i| maxima (exact in fp32)
 #pragma unroll
       for (int i = 0; i < E; ++i) {
-        m[0] = fmax(m[0], abs_or_inf((double)z1[i].x));
-        m[1] = fmax(m[1], abs_or_inf((double)z1[i].y));
-        m[2] = fmax(m[2], abs_or_inf((double)z2[i].x));
+        m[0] = fmaxf(m[0], abs_or_inf(z1[i].x));
+        m[1] = fmaxf(m[1], abs_or_inf(z1[i].y));
+        m[2] = fmaxf(m[2], abs_or_inf(z2[i].x));
       }
-      block_max_to<3>(m, f.stats, reinterpret_cast<double*>(smem_raw));
+      double md[3] = {(double)m[0], (double)m[1], (double)m[2]};
+      block_max_to<3>(md, f.stats, reinterpret_cast<double*>(smem_raw));
     }
     if (flags & kLineMaxOnly) return;
     cx<T> z3[E];
@@ -149,14 +151,15 @@ __global__ void __launch_bounds__(G3<N>::g.lineNT, 1) kc_line3d(Flow<float> f, i
     load_pair(z3, 4, 5);
     fft_ct<E, true, NT, Lay, G.line>(z3, lds, tw_rev, tid);
     if (flags & kLineWantMax) {
-      double m[3] = {0.0, 0.0, 0.0};
+      float m[3] = {0.f, 0.f, 0.f};  // |u_i| maxima (exact in fp32)
 #pragma unroll
       for (int i = 0; i < E; ++i) {
-        m[0] = fmax(m[0], abs_or_inf((double)z1[i].x));
-        m[1] = fmax(m[1], abs_or_inf((double)z1[i].y));
-        m[2] = fmax(m[2], abs_or_inf((double)z2[i].x));
+        m[0] = fmaxf(m[0], abs_or_inf(z1[i].x));
+        m[1] = fmaxf(m[1], abs_or_inf(z1[i].y));
+        m[2] = fmaxf(m[2], abs_or_inf(z2[i].x));
       }
-      block_max_to<3>(m, f.stats, reinterpret_cast<double*>(smem_raw));
+      double md[3] = {(double)m[0], (double)m[1], (double)m[2]};
+      block_max_to<3>(md, f.stats, reinterpret_cast<double*>(smem_raw));
     }
     if (flags & kLineMaxOnly) return;
 #pragma unroll
@@ -178,6 +181,7 @@ __global__ void __launch_bounds__(G3<N>::g.lineNT, 1) kc_line3d(Flow<float> f, i
 // two real fields' spectra, transform along y.  Inverse: transform along y, pack pairs.
 template <int N, bool ROT, bool INV>
 __global__ void __launch_bounds__(G3<N>::g.yNT, 1) kc_ypass3d(Flow<float> f) {
+  if (flow_inactive(f.prm, f.nh)) return;
   using T = float;
   constexpr Geo3D G = G3<N>::g;
   constexpr int E = G.yE, NT = G.yNT, CT = G.T, L = N, LDW = G.ldw, NJT = LDW / CT;
@@ -252,6 +256,7 @@ __global__ void __launch_bounds__(G3<N>::g.yNT, 1) kc_ypass3d(Flow<float> f) {
 // ------------------------------------------------------------------ x pass with physics
 template <int N, bool ROT, int stage>
 __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
+  if (flow_inactive(f.prm, f.nh)) return;
   using T = float;
   constexpr Geo3D G = G3<N>::g;
   constexpr int E = G.xE, NT = G.xNT, CT = G.T, L = N, JM = G.jm, LDW = G.ldw, SP = G.sp;

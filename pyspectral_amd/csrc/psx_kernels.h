@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "psx_drv.h"
 #include "psx_stage.h"
 
 namespace psx {
@@ -58,6 +59,11 @@ hipError_t launch_ct2d(const Flow<float>& f, int which, int arg, hipStream_t st)
 // 1 y forward, 2 x pass with physics (arg = stage), 3 y inverse.
 bool ct3d_supported(int n);
 hipError_t launch_ct3d(const Flow<float>& f, int which, int arg, hipStream_t st);
+
+// Device controller (psx_kdrv.hip).  which: 0 before the first stage's transforms, 1 after
+// the last stage, 2 record reduction, 3 / 4 mark the flows' parameter blocks active /
+// inactive.
+hipError_t launch_drv(int which, const DrvMember* mem, int count, hipStream_t st);
 
 // Utility / diagnostics kernels (psx_kutil.hip).
 hipError_t launch_params(double* prm, int nh, double nu, double dt, double alpha, double amp,

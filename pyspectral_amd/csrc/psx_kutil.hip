@@ -11,23 +11,7 @@ namespace psx {
 
 __global__ void k_params(double* prm, int nh, double nu, double dt, double alpha, double amp,
                          double klo2, double khi2, double ka2) {
-  for (int m = threadIdx.x; m < nh; m += blockDim.x) {
-    const double dm = (double)m;
-    prm[m] = exp(-nu * dm * dm * dt);
-    prm[nh + m] = exp(-nu * dm * dm * 0.5 * dt);
-  }
-  if (threadIdx.x == 0) {
-    double* s = prm + 2 * nh;
-    s[PrmLayout::kDt] = dt;
-    s[PrmLayout::kEa1] = exp(-alpha * dt);
-    s[PrmLayout::kEah] = exp(-alpha * 0.5 * dt);
-    s[PrmLayout::kEb1] = exp(amp * dt);
-    s[PrmLayout::kEbh] = exp(amp * 0.5 * dt);
-    s[PrmLayout::kKlo2] = klo2;
-    s[PrmLayout::kKhi2] = khi2;
-    s[PrmLayout::kKa2] = ka2;
-    s[PrmLayout::kBand] = amp != 0.0 ? 1.0 : 0.0;
-  }
+  write_params(prm, nh, nu, dt, alpha, amp, klo2, khi2, ka2, threadIdx.x, blockDim.x);
 }
 
 hipError_t launch_params(double* prm, int nh, double nu, double dt, double alpha, double amp,

@@ -155,5 +155,42 @@ __device__ __forceinline__ void block_max_to(double (&m)[NV],
 __device__ __forceinline__ double abs_or_inf(double a) {
   return __builtin_isnan(a) ? __builtin_inf() : fabs(a);
 }
+__device__ __forceinline__ float abs_or_inf(float a) {
+  return __builtin_isnan(a) ? __builtin_inff() : fabsf(a);
+}
+
+// Fill a flow's FP64 parameter block (PrmLayout) for step dt and band growth rate amp, with
+// the threads [tid, nthreads) of the caller: t_1[m] = exp(-nu m^2 dt), t_1/2[m] =
+// exp(-nu m^2 dt/2) (src/pyspectral/backends/_fused_common.py:10-19) and the drag/band
+// scalars.  The flow is marked active.
+__device__ __forceinline__ void write_params(double* prm, int nh, double nu, double dt,
+                                             double alpha, double amp, double klo2,
+                                             double khi2, double ka2, int tid, int nthreads) {
+  for (int m = tid; m < nh; m += nthreads) {
+    const double dm = (double)m;
+    prm[m] = exp(-nu * dm * dm * dt);
+    prm[nh + m] = exp(-nu * dm * dm * 0.5 * dt);
+  }
+  if (tid == 0) {
+    double* s = prm + 2 * nh;
+    s[PrmLayout::kDt] = dt;
+    s[PrmLayout::kEa1] = exp(-alpha * dt);
+    s[PrmLayout::kEah] = exp(-alpha * 0.5 * dt);
+    s[PrmLayout::kEb1] = exp(amp * dt);
+    s[PrmLayout::kEbh] = exp(amp * 0.5 * dt);
+    s[PrmLayout::kKlo2] = klo2;
+    s[PrmLayout::kKhi2] = khi2;
+    s[PrmLayout::kKa2] = ka2;
+    s[PrmLayout::kBand] = amp != 0.0 ? 1.0 : 0.0;
+    s[PrmLayout::kActive] = 1.0;
+  }
+}
+
+// Stage kernels return at once for a flow whose parameter block marks it inactive: a run of
+// the device controller that has reached its end turns the graphs queued behind it into
+// no-ops.
+__device__ __forceinline__ bool flow_inactive(const double* prm, int nh) {
+  return prm[2 * nh + PrmLayout::kActive] == 0.0;
+}
 
 }  // namespace psx
