@@ -95,7 +95,8 @@ int psx_step_graph(psx_ctx* c, void* stream);
 #define PSX_KIND_OUTER_INV 3 /* 2D four-step outer inverse pass */
 #define PSX_KIND_MID_FWD 4   /* 3D middle-axis forward pass */
 #define PSX_KIND_MID_INV 5   /* 3D middle-axis inverse pass */
-#define PSX_NKINDS 6
+#define PSX_KIND_CTRL 6      /* device time-loop controller and its record reduction */
+#define PSX_NKINDS 7
 
 /* Per-launch GPU timing of eager steps with HIP events on the caller's stream: between
  * psx_timing_begin and psx_timing_end every psx_step records an event before each launch;
@@ -166,6 +167,44 @@ int psx_run_peek(psx_ctx* c, int lag, psx_run_state* out);
 int psx_run_poll(psx_ctx* c, psx_run_state* out, double* rec, double* steps, void* stream);
 /* End a run: the context steps eagerly again (psx_set_step must follow before psx_step). */
 int psx_run_end(psx_ctx* c, void* stream);
+
+/* ---- ensembles of 2D flows ------------------------------------------------------------
+ * Independent 2D fp32 Basdevant flows of several sizes (powers of two, 64..2048) advanced
+ * together: each phase of a stage is one persistent launch over the tiles of every flow.
+ * Each flow follows the single-flow semantics above (the reference has no ensembles; per
+ * flow this replaces src/pyspectral/backends/cupy_cuda.py:175-195 and, for the time loop,
+ * simulate.py:40-96).  Every member has its own u0/acc/work buffers (sizes from
+ * psx_ens_query_layout), parameters, step, forcing amplitude and controller state. */
+typedef struct psx_ens psx_ens;
+
+/* Compiled spectral-pass variants for member size n (first = default) into out[0..cap);
+ * returns their number (0: n unsupported). */
+int psx_ens_variants(int n, int* out, int cap);
+/* Buffer sizes of one member planned with spectral variant `variant` (-1: default). */
+int psx_ens_query_layout(const psx_flow_params* p, int variant, psx_layout* out);
+/* Create an ensemble of `count` members borrowing their buffers. */
+int psx_ens_create(int count, const psx_flow_params* params, const int* variants,
+                   void* const* u0, void* const* acc, void* const* work, psx_ens** out);
+int psx_ens_destroy(psx_ens* e);
+/* Persistent work-groups per compute unit of every phase launch (1..16, default 2). */
+int psx_ens_set_grid(psx_ens* e, int wg_per_cu);
+/* First-stage inputs of every member from its state (after loading the states). */
+int psx_ens_prepare(psx_ens* e, void* stream);
+/* Per-member step and band growth rate (arrays of count). */
+int psx_ens_set_step(psx_ens* e, const double* dt, const double* amp, void* stream);
+/* One IF-RK4 step of every member, eagerly / replayed from a graph captured once. */
+int psx_ens_step(psx_ens* e, void* stream);
+int psx_ens_step_graph(psx_ens* e, void* stream);
+/* Per-launch event timing of eager steps, as psx_timing_begin / psx_timing_end. */
+int psx_ens_timing_begin(psx_ens* e);
+int psx_ens_timing_end(psx_ens* e, void* stream, double* ms, int* count);
+/* Device time loop of every member (arrays of count psx_run_params / psx_run_state; the ring
+ * capacities and the CFL-or-fixed choice are shared); rings are member-major. */
+int psx_ens_run_begin(psx_ens* e, const psx_run_params* rp, void* stream);
+int psx_ens_run_chunk(psx_ens* e, int nsteps, int use_graph, void* stream);
+int psx_ens_run_peek(psx_ens* e, int lag, psx_run_state* out);
+int psx_ens_run_poll(psx_ens* e, psx_run_state* out, double* rec, double* steps, void* stream);
+int psx_ens_run_end(psx_ens* e, void* stream);
 
 /* Message of the last failure on this thread. */
 const char* psx_last_error(void);

@@ -62,11 +62,28 @@ SIGNATURES = {
     "psx_run_peek": (c_int, [c_void_p, c_int, POINTER(RunState)]),
     "psx_run_poll": (c_int, [c_void_p, POINTER(RunState), c_void_p, c_void_p, c_void_p]),
     "psx_run_end": (c_int, [c_void_p, c_void_p]),
+    "psx_ens_variants": (c_int, [c_int, POINTER(c_int), c_int]),
+    "psx_ens_query_layout": (c_int, [POINTER(FlowParams), c_int, POINTER(Layout)]),
+    "psx_ens_create": (c_int, [c_int, POINTER(FlowParams), POINTER(c_int), POINTER(c_void_p),
+                               POINTER(c_void_p), POINTER(c_void_p), POINTER(c_void_p)]),
+    "psx_ens_destroy": (c_int, [c_void_p]),
+    "psx_ens_set_grid": (c_int, [c_void_p, c_int]),
+    "psx_ens_prepare": (c_int, [c_void_p, c_void_p]),
+    "psx_ens_set_step": (c_int, [c_void_p, POINTER(c_double), POINTER(c_double), c_void_p]),
+    "psx_ens_step": (c_int, [c_void_p, c_void_p]),
+    "psx_ens_step_graph": (c_int, [c_void_p, c_void_p]),
+    "psx_ens_timing_begin": (c_int, [c_void_p]),
+    "psx_ens_timing_end": (c_int, [c_void_p, c_void_p, POINTER(c_double), POINTER(c_int)]),
+    "psx_ens_run_begin": (c_int, [c_void_p, POINTER(RunParams), c_void_p]),
+    "psx_ens_run_chunk": (c_int, [c_void_p, c_int, c_int, c_void_p]),
+    "psx_ens_run_peek": (c_int, [c_void_p, c_int, POINTER(RunState)]),
+    "psx_ens_run_poll": (c_int, [c_void_p, POINTER(RunState), c_void_p, c_void_p, c_void_p]),
+    "psx_ens_run_end": (c_int, [c_void_p, c_void_p]),
     "psx_last_error": (c_char_p, []),
     "psx_device_info": (c_int, [c_char_p, c_int]),
 }
 
-KINDS = ("line", "outer_fwd", "spec", "outer_inv", "mid_fwd", "mid_inv")  # PSX_KIND_*
+KINDS = ("line", "outer_fwd", "spec", "outer_inv", "mid_fwd", "mid_inv", "ctrl")  # PSX_KIND_*
 
 _LIB: ctypes.CDLL | None = None
 

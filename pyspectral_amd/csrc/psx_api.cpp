@@ -11,30 +11,12 @@
 #include <vector>
 
 #include "psx_geo.h"
+#include "psx_host.h"
 #include "psx_kernels.h"
 
 using namespace psx;
 
 namespace {
-
-thread_local std::string g_err;
-
-int fail(const char* fmt, ...) {
-  char buf[512];
-  va_list ap;
-  va_start(ap, fmt);
-  vsnprintf(buf, sizeof buf, fmt, ap);
-  va_end(ap);
-  g_err = buf;
-  return 1;
-}
-
-#define PSX_HIP(expr)                                                                   \
-  do {                                                                                  \
-    hipError_t e_ = (expr);                                                             \
-    if (e_ != hipSuccess) return fail("%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                                      __FILE__, __LINE__);                              \
-  } while (0)
 
 // ------------------------------------------------------------------------------- planning
 struct Factors {
@@ -316,56 +298,6 @@ int plan_flow(const psx_flow_params& p, psx_ctx* c, Flow<T>* f) {
 }
 
 template <typename T>
-int make_twiddles(int n, void** out) {
-  std::vector<cx<T>> h(n);
-  for (int i = 0; i < n; ++i) {
-    const long double ang = 2.0L * 3.14159265358979323846264338327950288L * i / n;
-    h[i].x = (T)(double)std::cos(ang);
-    h[i].y = (T)(double)(-std::sin(ang));
-  }
-  PSX_HIP(hipMalloc(out, n * sizeof(cx<T>)));
-  PSX_HIP(hipMemcpy(*out, h.data(), n * sizeof(cx<T>), hipMemcpyHostToDevice));
-  return 0;
-}
-
-// Append the per-pass twiddle tables of plan p (forward and reversed radix order) to buf and
-// record their offsets in p->twoff.
-template <typename T>
-void append_pass_twiddles(FftPlan* p, std::vector<cx<T>>& buf) {
-  const long double two_pi = 6.283185307179586476925286766559005768L;
-  for (int order = 0; order < 2; ++order) {
-    int Ns = 1;
-    for (int q = 0; q < p->np; ++q) {
-      const int R = order == 0 ? p->rad[q] : p->rad[p->np - 1 - q];
-      p->twoff[order][q] = (int)buf.size();
-      if (Ns > 1) {
-        for (int m = 1; m < R; ++m)
-          for (int k = 0; k < Ns; ++k) {
-            const long double ang = two_pi * (long double)(m * k) / (long double)(Ns * R);
-            cx<T> w;
-            w.x = (T)(double)std::cos(ang);
-            w.y = (T)(double)(-std::sin(ang));
-            buf.push_back(w);
-          }
-      }
-      Ns *= R;
-    }
-  }
-}
-
-template <typename T>
-int make_pass_twiddles(Flow<T>* f, void** out) {
-  std::vector<cx<T>> buf;
-  FftPlan* plans[4] = {&f->prow, &f->pcol, &f->pa, &f->py};
-  for (FftPlan* p : plans)
-    if (p->np > 0) append_pass_twiddles<T>(p, buf);
-  if (buf.empty()) buf.push_back(cx<T>{(T)1, (T)0});
-  PSX_HIP(hipMalloc(out, buf.size() * sizeof(cx<T>)));
-  PSX_HIP(hipMemcpy(*out, buf.data(), buf.size() * sizeof(cx<T>), hipMemcpyHostToDevice));
-  return 0;
-}
-
-template <typename T>
 Flow<T>& flow(psx_ctx* c);
 template <>
 Flow<float>& flow<float>(psx_ctx* c) { return c->f32; }
@@ -443,7 +375,10 @@ int enqueue_step(psx_ctx* c, hipStream_t st, int drv = 0) {
       if (f.dim == 2) PSX_HIP(launch_line2d<T>(f, c->line, lflags, st));
       else PSX_HIP(launch_line3d<T>(f, c->line, lflags, st));
     }
-    if (s == 0 && drv) PSX_HIP(launch_drv(0, c->dmem, 1, st));
+    if (s == 0 && drv) {
+      PSX_MARK(PSX_KIND_CTRL);
+      PSX_HIP(launch_drv(0, c->dmem, 1, st));
+    }
     if constexpr (sizeof(T) == 4) {
       if (ct && f.dim == 3) {
         PSX_MARK(PSX_KIND_MID_FWD);
@@ -488,6 +423,7 @@ int enqueue_step(psx_ctx* c, hipStream_t st, int drv = 0) {
     }
   }
   if (drv) {
+    PSX_MARK(PSX_KIND_CTRL);
     PSX_HIP(launch_drv(1, c->dmem, 1, st));
     PSX_HIP(launch_drv(2, c->dmem, 1, st));
   }
@@ -797,30 +733,7 @@ int psx_run_begin(psx_ctx* c, const psx_run_params* rp, void* stream) {
   m.fp64 = c->fp64;
   m.cap_rec = rp->cap_rec;
   m.cap_step = rp->cap_step;
-  DrvState s{};
-  s.t = rp->t0;
-  s.t_end = rp->t_end;
-  s.eps = 1e-12 * std::fmax(1.0, std::isfinite(rp->t_end) ? rp->t_end : 1.0);
-  s.dt_fixed = rp->dt_fixed > 0.0 ? rp->dt_fixed : 0.0;
-  s.cfl = rp->cfl;
-  s.dt_max = rp->dt_max;
-  s.dx = 2.0 * M_PI / c->p.n;
-  s.diag_dt = rp->diag_dt > 0.0 ? rp->diag_dt : 0.0;
-  s.next_diag = !(rp->diag_dt > 0.0)          ? INFINITY
-                : std::isfinite(rp->next_diag) ? rp->next_diag
-                                               : rp->t0 + rp->diag_dt;
-  s.power = rp->power;
-  s.amp = rp->amp0;
-  s.cur_dt = s.dt_fixed;
-  s.h = 0.0;
-  s.last_rec_t = -INFINITY;
-  s.k = 0;
-  s.kmax = rp->kmax > 0 ? rp->kmax : (long long)9e18;
-  s.cfl_every = rp->cfl_every;
-  s.nrec = 0;
-  s.done = !(s.t < s.t_end - s.eps) || s.kmax <= 0;
-  s.active = !s.done;
-  s.record = rp->record_start ? 1 : 0;
+  const DrvState s = drv_initial_state(*rp, c->p.n);
   c->hmem = m;
   c->run_mode = s.dt_fixed > 0.0 ? 1 : 2;
   PSX_HIP(hipMemcpyAsync(c->dmem, &c->hmem, sizeof(DrvMember), hipMemcpyHostToDevice, st));
@@ -900,14 +813,7 @@ int psx_run_peek(psx_ctx* c, int lag, psx_run_state* out) {
                 c->nchunk);
   const int slot = (int)((c->nchunk - 1 - lag) % psx_ctx::kSlots);
   PSX_HIP(hipEventSynchronize(c->slot_ev[slot]));
-  const DrvState& s = c->hslots[slot];
-  out->t = s.t;
-  out->amp = s.amp;
-  out->cur_dt = s.cur_dt;
-  out->next_diag = s.next_diag;
-  out->k = s.k;
-  out->nrec = s.nrec;
-  out->done = s.done;
+  drv_export(c->hslots[slot], out);
   return 0;
 }
 
@@ -924,13 +830,7 @@ int psx_run_poll(psx_ctx* c, psx_run_state* out, double* rec, double* steps, voi
     PSX_HIP(hipMemcpyAsync(steps, c->hmem.steps, 2 * (size_t)c->hmem.cap_step * sizeof(double),
                            hipMemcpyDeviceToHost, st));
   PSX_HIP(hipStreamSynchronize(st));
-  out->t = s.t;
-  out->amp = s.amp;
-  out->cur_dt = s.cur_dt;
-  out->next_diag = s.next_diag;
-  out->k = s.k;
-  out->nrec = s.nrec;
-  out->done = s.done;
+  drv_export(s, out);
   return 0;
 }
 

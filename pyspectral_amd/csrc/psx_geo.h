@@ -82,6 +82,7 @@ struct Geo2D {
   int lineE = 0, lineNT = 0, lineT = 0;
   int specE = 0, specNT = 0;
   int outE = 0, outNT = 0;
+  int outS = 1;  // outer-pass tiles per work-group (ensemble kernels)
   CtPlan line{}, spec{}, outer{};
 };
 
@@ -150,6 +151,65 @@ constexpr Geo2D geo2d(int n, bool fp64) {
 
 // Grid sizes with compile-time specialised 2D fp32 kernels.
 #define PSX_CT2D_SIZES(X) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
+
+// ---------------------------------------------------------------------------- ensembles
+// Ensemble kernels run every phase as one launch of kEnsNT-thread work-groups over the tiles
+// of flows of several sizes, so every geometry below gives each kernel kind kEnsNT threads
+// per work-group (the outer four-step pass as outS independent tiles of outNT threads).
+// Spectral-pass variants (the autotuner's choices):
+//   kEnsS16, kEnsS8, kEnsS4   one pass over whole columns, column tiles of 16, 8 or 4;
+//   kEnsF256, kEnsF128        four-step split with inner length 256 or 128 (tiles of 16
+//                             columns) and an outer pass of n/256 or n/128 points held in
+//                             the registers of one thread.
+constexpr int kEnsNT = 256;
+enum : int { kEnsS16 = 0, kEnsS8 = 1, kEnsS4 = 2, kEnsF256 = 3, kEnsF128 = 4, kEnsNVar = 5 };
+
+constexpr Geo2D geo2d_ens(int n, int var) {
+  Geo2D g{};
+  if (n < 64 || (n & (n - 1)) || var < 0 || var >= kEnsNVar) return g;
+  g.n = n;
+  g.jm = ct_retained(n);
+  g.sp = ct_state_pitch(n, false);
+  // line kernel: 16 elements per thread, kEnsNT * 16 / n whole rows per work-group
+  g.lineE = 16;
+  g.lineNT = kEnsNT;
+  g.lineT = kEnsNT * 16 / n;
+  g.line = ct_plan(n, 16);
+  if (g.lineT < 1 || g.line.np == 0) return g;
+  int T = 16, L = n;
+  if (var == kEnsS8) T = 8;
+  if (var == kEnsS4) T = 4;
+  if (var == kEnsF256) L = 256;
+  if (var == kEnsF128) L = 128;
+  if (L > n) return g;
+  const int E = L * T / kEnsNT;
+  if (E < 4 || E > 16) return g;
+  g.colT = T;
+  g.ldw = (g.jm + T - 1) / T * T;
+  g.n2 = L;
+  g.n1 = n / L;
+  g.specE = E;
+  g.specNT = kEnsNT;
+  g.spec = ct_plan(L, E);
+  if (g.spec.np == 0 || L * T * 8 > 65536) return g;
+  if (g.n1 > 1) {
+    if (g.n1 > 16) return g;
+    g.outE = g.n1;
+    g.outNT = T;  // one thread per column, all n1 points in registers
+    g.outS = kEnsNT / g.outNT;
+    g.outer = ct_plan(g.n1, g.n1);
+    if (g.outer.np != 1) return g;
+    if ((g.n2 * (g.ldw / T)) % g.outS) return g;
+  }
+  g.ok = true;
+  return g;
+}
+
+// Ensemble member sizes and the spectral variants compiled for each (first = default).
+#define PSX_ENS_VARIANTS(X)                                                        \
+  X(64, kEnsS16) X(128, kEnsS16) X(128, kEnsS8) X(256, kEnsS16) X(256, kEnsF128) \
+  X(512, kEnsS8) X(512, kEnsF128) X(1024, kEnsS4) X(1024, kEnsF128)              \
+  X(2048, kEnsF256) X(2048, kEnsF128)
 
 // Launch geometry of the 3D kernels (fp32): z lines, y and x column passes over tiles of
 // T columns of the retained half axis.

@@ -1,18 +1,8 @@
 // 2D fp32 stage kernels specialised per grid size (PSX_CT2D_SIZES): the same algorithms as
 // psx_k2d.hip with the plans, launch shapes and HBM strides fixed at compile time by the
-// shared planner geo2d() (psx_geo.h).
-//
-// Work-field convention of this path at the real-axis boundary: the line kernel reads and
-// writes each row's complex line spectrum Z directly, Z(e) = U(e) + i V(e) for the packed pair
-// of real fields, stored as two half spectra: field 0 holds Z(j) and field 1 holds Z(n - j)
-// for the retained j < jm (field 1, column 0 unused).  The column kernels pack (U, V) into Z
-// after their inverse transform (Z(j) = U + iV, Z(n-j) = conj(U) + i conj(V), imaginary parts
-// of the j = 0 column dropped as a c2r does) and split Z back into the two real fields'
-// spectra before their forward transform (P = (Z(j) + conj Z(n-j))/2,
-// Q = -i (Z(j) - conj Z(n-j))/2).  The line kernel then needs one load and one store per
-// element and no LDS pass of its own.
-#include "psx_fft_ct.h"
-#include "psx_kernels.h"
+// shared planner geo2d() (psx_geo.h); one work-group per tile.  The bodies and the packed
+// line-spectrum convention are in psx_k2d_body.h.
+#include "psx_k2d_body.h"
 
 namespace psx {
 
@@ -22,266 +12,26 @@ struct G2 {
   static_assert(g.ok, "no 2D plan for this size");
 };
 
-// --------------------------------------------------------------------------- line kernel
 template <int N>
 __global__ void __launch_bounds__(G2<N>::g.lineNT, 1) kc_line2d(Flow<float> f, int flags) {
   if (flow_inactive(f.prm, f.nh)) return;
-  using T = float;
-  constexpr Geo2D G = G2<N>::g;
-  constexpr int E = G.lineE, NT = G.lineNT, RT = G.lineT, L = N, JM = G.jm, LDW = G.ldw;
-  constexpr long long FS = (long long)N * LDW;
-  using Lay = RowsCT<L, RT>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
-  const int tid = threadIdx.x;
-  const int row0 = blockIdx.x * RT;
-  const BufC32 zb(f.W, (unsigned long long)(2 * FS) * 8);
-  const cx<T>* tw_fwd = f.twp + f.prow.twoff[0][0];
-  const cx<T>* tw_rev = f.twp + f.prow.twoff[1][0];
-  // byte offset of element e of row-in-tile t: field 0 for e < JM, field 1 (mirrored) for
-  // e > L - JM, out of range (zero / dropped) in between
-  auto zoff = [&](int e, int t) -> unsigned {
-    const unsigned base = (unsigned)(row0 + t) * LDW;
-    return e < JM ? (base + e) * 8u
-                  : (e > L - JM ? ((unsigned)FS + base + (L - e)) * 8u : kOob);
-  };
-
-  cx<T> v[E];
-  {
-    constexpr int R = ct_radix(G.line, true, 0);
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      int e, t;
-      ct_elem<R, L, NT, Lay>(i, tid, e, t);
-      v[i] = zb.load(zoff(e, t));
-    }
-  }
-  fft_ct<E, true, NT, Lay, G.line>(v, lds, tw_rev, tid);
-
-  if (flags & kLineWantMax) {
-    float m[2] = {0.f, 0.f};  // |u_i| maxima (exact in fp32)
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      m[0] = fmaxf(m[0], abs_or_inf(v[i].x));
-      m[1] = fmaxf(m[1], abs_or_inf(v[i].y));
-    }
-    double md[2] = {(double)m[0], (double)m[1]};
-      block_max_to<2>(md, f.stats, reinterpret_cast<double*>(smem_raw));
-  }
-  if (flags & kLineMaxOnly) return;
-
-#pragma unroll
-  for (int i = 0; i < E; ++i) {
-    const T u = v[i].x, w = v[i].y;
-    v[i] = mkc((w - u) * (w + u), u * w);
-  }
-  fft_ct<E, false, NT, Lay, G.line>(v, lds, tw_fwd, tid);
-  {
-    constexpr int R = ct_radix(G.line, false, G.line.np - 1);
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      int e, t;
-      ct_elem<R, L, NT, Lay>(i, tid, e, t);
-      zb.store(zoff(e, t), v[i]);
-    }
-  }
+  body_line2d<G2<N>::g>(f, blockIdx.x, threadIdx.x, flags, smem_raw);
 }
 
-// Split of the two real fields' spectra from a packed line spectrum at column j.
-__device__ __forceinline__ void z_split(cx<float> z, cx<float> zm, cx<float>& a, cx<float>& b) {
-  const cx<float> zc = conjg(zm);
-  a = cscl(0.5f, cadd(z, zc));
-  b = cscl(0.5f, mul_mi(csub(z, zc)));
-}
-
-// Packed line spectrum from the two real fields' spectra at column j.
-__device__ __forceinline__ void z_pack(int j, cx<float> u, cx<float> v, cx<float>& z,
-                                       cx<float>& zm) {
-  if (j == 0) {
-    z = mkc(u.x, v.x);
-    zm = z;
-  } else {
-    z = mkc(u.x - v.y, u.y + v.x);
-    zm = mkc(u.x + v.y, v.x - u.y);
-  }
-}
-
-// ----------------------------------------------------------------------- spectral kernel
 template <int N, int stage>
 __global__ void __launch_bounds__(G2<N>::g.specNT, 2) kc_spec2d(Flow<float> f) {
   if (flow_inactive(f.prm, f.nh)) return;
-  using T = float;
-  constexpr Geo2D G = G2<N>::g;
-  constexpr bool FOUR = G.n1 > 1;
-  constexpr int E = G.specE, NT = G.specNT, CT = G.colT, L = G.n2, N1 = G.n1, JM = G.jm;
-  constexpr int LDW = G.ldw, NH = N / 2 + 1, SP = G.sp;
-  constexpr long long FS = (long long)N * LDW;
-  constexpr int NJT = LDW / CT;
-  using Lay = ColsCT<CT>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
-  const int tid = threadIdx.x;
-  const int grp = blockIdx.x / NJT;
-  const int j0 = (blockIdx.x - grp * NJT) * CT;
-  const int eo = FOUR ? grp * L : 0;
-  cx<T>* W0 = f.W;
-  cx<T>* W1 = f.W + FS;
-  const cx<T>* tw_fwd = f.twp + f.pcol.twoff[0][0];
-  const cx<T>* tw_rev = f.twp + f.pcol.twoff[1][0];
-
-  cx<T> va[E], vb[E];
-  if constexpr (stage != kStagePre) {
-    constexpr int R = ct_radix(G.spec, false, 0);
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      int e, t;
-      ct_elem<R, L, NT, Lay>(i, tid, e, t);
-      const long long a = (long long)(eo + e) * LDW + j0 + t;
-      if constexpr (FOUR) {
-        va[i] = W0[a];
-        vb[i] = W1[a];
-      } else {
-        const cx<T> z = W0[a];
-        const cx<T> zm = W1[a];
-        z_split(z, j0 + t == 0 ? z : zm, va[i], vb[i]);
-      }
-    }
-  }
-  // State and accumulator of this thread's modes, loaded before the transforms so their
-  // latency hides behind them; modes outside the 2/3 mask read zero and are never written.
-  constexpr int RP = ct_radix(G.spec, false, G.spec.np - 1);
-  const BufC32 ub(f.u0, (unsigned long long)N * SP * 8);
-  const BufC32 ab(f.acc, (unsigned long long)N * SP * 8);
-  unsigned soff[E];
-  cx<T> uu[E], aa[E];
-#pragma unroll
-  for (int i = 0; i < E; ++i) {
-    int e, t;
-    ct_elem<RP, L, NT, Lay>(i, tid, e, t);
-    const int k = FOUR ? grp + N1 * e : e;
-    const int kx = k < N / 2 ? k : k - N;
-    const int ax = kx < 0 ? -kx : kx;
-    const int ky = j0 + t;
-    soff[i] = (3 * ax < N && ky < JM) ? (unsigned)(k * SP + ky) * 8u : kOob;
-    uu[i] = ub.load(soff[i]);
-  }
-  if constexpr (stage >= 1) {
-#pragma unroll
-    for (int i = 0; i < E; ++i) aa[i] = ab.load(soff[i]);
-  } else {
-#pragma unroll
-    for (int i = 0; i < E; ++i) aa[i] = czero<T>();
-  }
-  if constexpr (stage != kStagePre) {
-    fft_ct<E, false, NT, Lay, G.spec>(va, lds, tw_fwd, tid);
-    fft_ct<E, false, NT, Lay, G.spec>(vb, lds, tw_fwd, tid);
-  }
-  {
-    const IfScalars s = load_if_scalars(f.prm, NH);
-    const T dt = (T)f.prm[2 * NH + PrmLayout::kDt];
-    const T scale = f.scale;
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      int e, t;
-      ct_elem<RP, L, NT, Lay>(i, tid, e, t);
-      const int k = FOUR ? grp + N1 * e : e;
-      const int kx = k < N / 2 ? k : k - N;
-      const int ky = j0 + t;
-      const int ax = kx < 0 ? -kx : kx;
-      const int k2 = kx * kx + ky * ky;
-      cx<T> nxt = uu[i];
-      if constexpr (stage != kStagePre) {
-        const T fx = (T)kx, fy = (T)ky;
-        const cx<T> Nl = cadd(cscl(fx * fy, va[i]), cscl(fx * fx - fy * fy, vb[i]));
-        double E1, Eh;
-        if_factors(f.prm, NH, s, ax, ky, -1, k2, E1, Eh);
-        cx<T> acc = aa[i];
-        nxt = rk_value<stage, T>(dt, (T)E1, (T)Eh, Nl, uu[i], acc);
-        if constexpr (stage < 3) ab.store(soff[i], acc);
-        else ub.store(soff[i], nxt);
-      }
-      if (soff[i] == kOob) nxt = czero<T>();  // 2/3 mask: no nonlinear term outside it
-      const T ik2 = k2 > 0 ? scale * __builtin_amdgcn_rcpf((T)k2) : (T)0;
-      const cx<T> psi = cscl(ik2, nxt);
-      va[i] = cscl((T)ky, mul_i(psi));
-      vb[i] = cscl((T)kx, mul_mi(psi));
-    }
-  }
-  constexpr int RS = ct_radix(G.spec, true, G.spec.np - 1);
-  fft_ct<E, true, NT, Lay, G.spec>(va, lds, tw_rev, tid);
-  fft_ct<E, true, NT, Lay, G.spec>(vb, lds, tw_rev, tid);
-#pragma unroll
-  for (int i = 0; i < E; ++i) {
-    int e, t;
-    ct_elem<RS, L, NT, Lay>(i, tid, e, t);
-    const long long a = (long long)(eo + e) * LDW + j0 + t;
-    if constexpr (FOUR) {
-      const cx<T> w = f.tw[e * grp];
-      W0[a] = cmulc(va[i], w);
-      W1[a] = cmulc(vb[i], w);
-    } else {
-      cx<T> z, zm;
-      z_pack(j0 + t, va[i], vb[i], z, zm);
-      W0[a] = z;
-      W1[a] = zm;
-    }
-  }
+  body_spec2d<G2<N>::g, stage>(f, blockIdx.x, threadIdx.x, smem_raw);
 }
 
-// ------------------------------------------------------------------- outer four-step pass
 template <int N, bool INV>
 __global__ void __launch_bounds__(G2<N>::g.outNT > 0 ? G2<N>::g.outNT : 64, 1)
     kc_outer2d(Flow<float> f) {
   if (flow_inactive(f.prm, f.nh)) return;
-  using T = float;
-  constexpr Geo2D G = G2<N>::g;
-  if constexpr (G.n1 > 1) {
-    constexpr int E = G.outE, NT = G.outNT, CT = G.colT, L = G.n1, N2 = G.n2;
-    constexpr int LDW = G.ldw, NJT = LDW / CT;
-    constexpr long long FS = (long long)N * LDW;
-    using Lay = ColsCT<CT>;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
-    const int tid = threadIdx.x;
-    const int x2 = blockIdx.x / NJT;
-    const int j0 = (blockIdx.x - x2 * NJT) * CT;
-    const cx<T>* tw_o = f.twp + f.pa.twoff[INV ? 1 : 0][0];
-    constexpr int R0 = ct_radix(G.outer, INV, 0);
-    constexpr int RL = ct_radix(G.outer, INV, G.outer.np - 1);
-    cx<T> v0[E], v1[E];
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      int e, t;
-      ct_elem<R0, L, NT, Lay>(i, tid, e, t);
-      const long long a = (long long)(e * N2 + x2) * LDW + j0 + t;
-      if constexpr (INV) {
-        v0[i] = f.W[a];
-        v1[i] = f.W[FS + a];
-      } else {
-        const cx<T> z = f.W[a];
-        const cx<T> zm = f.W[FS + a];
-        z_split(z, j0 + t == 0 ? z : zm, v0[i], v1[i]);
-      }
-    }
-    fft_ct<E, INV, NT, Lay, G.outer>(v0, lds, tw_o, tid);
-    fft_ct<E, INV, NT, Lay, G.outer>(v1, lds, tw_o, tid);
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      int e, t;
-      ct_elem<RL, L, NT, Lay>(i, tid, e, t);
-      cx<T> z0 = v0[i], z1 = v1[i];
-      if constexpr (!INV) {
-        const cx<T> w = f.tw[e * x2];
-        z0 = cmul(z0, w);
-        z1 = cmul(z1, w);
-      } else {
-        z_pack(j0 + t, v0[i], v1[i], z0, z1);
-      }
-      const long long a = (long long)(e * N2 + x2) * LDW + j0 + t;
-      f.W[a] = z0;
-      f.W[FS + a] = z1;
-    }
-  }
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  body_outer2d<G2<N>::g, INV>(f, blockIdx.x, threadIdx.x, smem_raw);
 }
 
 // ------------------------------------------------------------------------------ launchers

@@ -1,0 +1,100 @@
+// Persistent multi-size ensemble kernels (psx_ens.h): one launch per phase for every flow
+// of an ensemble.  Work-group b handles items b, b + gridDim.x, ...; each item runs the body
+// of psx_k2d_body.h compiled for its flow's (size, spectral variant), so flows of different
+// sizes share one launch and the launch never waits on a smaller grid's tail.
+#include "psx_ens.h"
+#include "psx_k2d_body.h"
+
+namespace psx {
+
+template <int N, int V>
+struct GE {
+  static constexpr Geo2D g = geo2d_ens(N, V);
+  static_assert(g.ok, "no ensemble geometry for this size and variant");
+};
+
+template <int KIND, int ARG>
+__device__ __forceinline__ void ens_item(const EnsFlow& ef, int tile, int tid, int flags,
+                                         unsigned char* smem) {
+  switch (ef.code) {
+#define PSX_X(NV, VV)                                                                  \
+  case ens_code(NV, VV): {                                                             \
+    constexpr Geo2D G = GE<NV, VV>::g;                                                 \
+    if constexpr (KIND == 0) {                                                         \
+      body_line2d<G>(ef.f, tile, tid, flags, smem);                                    \
+    } else if constexpr (KIND == 1) {                                                  \
+      body_spec2d<G, ARG>(ef.f, tile, tid, smem);                                      \
+    } else if constexpr (G.n1 > 1) {                                                   \
+      const int sub = tid / G.outNT;                                                   \
+      body_outer2d<G, KIND == 3>(ef.f, tile * G.outS + sub, tid % G.outNT,             \
+                                 smem + (size_t)sub * G.n1 * G.colT * 8);              \
+    }                                                                                  \
+    break;                                                                             \
+  }
+    PSX_ENS_VARIANTS(PSX_X)
+#undef PSX_X
+    default:
+      break;
+  }
+}
+
+template <int KIND, int ARG>
+__global__ void __launch_bounds__(kEnsNT, 2)
+    ke_phase(const EnsFlow* __restrict__ flows, const EnsItem* __restrict__ items, int nitems,
+             int flags) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const EnsItem item = items[it];
+    const EnsFlow& ef = flows[item.member];
+    // the thread index is made opaque per item so that no body's index arithmetic is hoisted
+    // out of the loop (it would stay live across every variant's code)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    if (!flow_inactive(ef.f.prm, ef.f.nh))
+      ens_item<KIND, ARG>(ef, item.tile, tid, flags, smem_raw);
+    __syncthreads();  // the next item reuses the LDS
+  }
+}
+
+bool ens_compiled(int n, int var) {
+  switch (ens_code(n, var)) {
+#define PSX_X(NV, VV) case ens_code(NV, VV):
+    PSX_ENS_VARIANTS(PSX_X)
+#undef PSX_X
+    return true;
+    default:
+      return false;
+  }
+}
+
+template <int KIND, int ARG>
+static hipError_t ens_go(const EnsFlow* flows, const EnsItem* items, int nitems, int grid,
+                         size_t lds, int flags, hipStream_t st) {
+  auto fn = ke_phase<KIND, ARG>;
+  static const bool attr_ = set_lds_max((const void*)fn);
+  (void)attr_;
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(kEnsNT), lds, st, flows, items, nitems, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_ens(int kind, int arg, const EnsFlow* flows, const EnsItem* items, int nitems,
+                      int grid, size_t lds, hipStream_t st) {
+  if (nitems <= 0 || grid <= 0) return hipErrorInvalidValue;
+  switch (kind) {
+    case 0: return ens_go<0, 0>(flows, items, nitems, grid, lds, arg, st);
+    case 1:
+      switch (arg) {
+        case kStagePre: return ens_go<1, kStagePre>(flows, items, nitems, grid, lds, 0, st);
+        case 0: return ens_go<1, 0>(flows, items, nitems, grid, lds, 0, st);
+        case 1: return ens_go<1, 1>(flows, items, nitems, grid, lds, 0, st);
+        case 2: return ens_go<1, 2>(flows, items, nitems, grid, lds, 0, st);
+        case 3: return ens_go<1, 3>(flows, items, nitems, grid, lds, 0, st);
+        default: return hipErrorInvalidValue;
+      }
+    case 2: return ens_go<2, 0>(flows, items, nitems, grid, lds, 0, st);
+    case 3: return ens_go<3, 0>(flows, items, nitems, grid, lds, 0, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace psx
