@@ -13,7 +13,7 @@ bandwidth (the reference's definition, src/pyspectral/benchmark.py:437-443); a r
 object for the dominant kernel from HIP events around every launch of the timed region; and
 the CPU oracle (scipy.fft) timed on a bounded sample on this host.
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--n N] [--no-cpu] [--workload ensemble]
 """
 
 from __future__ import annotations
@@ -123,6 +123,121 @@ def cpu_baseline(n: int, state: np.ndarray, dt: float, budget_s: float = 30.0) -
             "s_per_step": t, "cpu": platform.processor() or platform.machine()}
 
 
+def ensemble_bench(a: argparse.Namespace) -> int:
+    """The batched forced ensemble (64 x 256^2 + 32 x 512^2 + 16 x 1024^2 + 4 x 2048^2, fp32,
+    constant-power band forcing 4 <= |k| <= 6 with P = 1e-3, drag 0.1, nu per size for
+    k_max eta = 1, CFL 0.5): autotuned plans, device-resident loop, one launch per phase."""
+    import torch
+
+    from pyspectral_amd import _lib, initial
+    from pyspectral_amd.ensemble import EnsembleSolver, autotune, forced_ensemble_configs
+
+    cfgs = forced_ensemble_configs()
+    npts = sum(c.n * c.n for c in cfgs)
+    states = [initial.random_phase_2d(c.n, k0=6.0, seed=1000 + m, dtype=np.complex64)
+              for m, c in enumerate(cfgs)]
+    t0 = time.perf_counter()
+    tuned = autotune(cfgs, nsteps=3)
+    tune_s = time.perf_counter() - t0
+    ens = EnsembleSolver(cfgs, tuned["variants"], wg_per_cu=tuned["wg_per_cu"])
+    ens.set_states(states)
+    run_kw = dict(cfl=0.5, dt_max=0.05, cfl_every=10, target_power=1e-3, chunk=10,
+                  record_start=False)
+    ens.run_device(math.inf, kmax=a.warmup, use_graph=False, **run_kw)
+    torch.cuda.synchronize()
+    lib, st = ens.lib, _lib.stream_ptr()
+    _lib.check(lib.psx_ens_timing_begin(ens.ctx))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    outs = ens.run_device(math.inf, kmax=a.steps, use_graph=False, **run_kw)
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / a.steps
+    nk = len(_lib.KINDS)
+    ms = (ctypes.c_double * nk)()
+    cnt = (ctypes.c_int * nk)()
+    _lib.check(lib.psx_ens_timing_end(ens.ctx, st, ms, cnt))
+    per_kernel = {_lib.KINDS[i]: {"ms_total": ms[i], "launches": cnt[i],
+                                  "us_per_launch": 1e3 * ms[i] / cnt[i]}
+                  for i in range(nk) if cnt[i]}
+    # the same loop from captured chunk graphs
+    ens.set_states(states)
+    ens.t = [0.0] * len(cfgs)
+    ens.run_device(math.inf, kmax=20, use_graph=True, **run_kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.run_device(math.inf, kmax=a.steps, use_graph=True, **run_kw)
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - t0) / a.steps
+    finite = all(math.isfinite(e) for o in outs for e in o["energy"] + o["dt"])
+
+    # algorithmic bytes of one launch of each phase: the sum over members
+    pb: dict[str, float] = {"line": 0.0, "spec": 0.0, "outer_fwd": 0.0, "outer_inv": 0.0}
+    for c, v in zip(cfgs, tuned["variants"]):
+        kb = kernel_bytes(c.n, (c.n - 1) // 3 + 1, 8)
+        pb["line"] += kb["line"]
+        pb["spec"] += kb["spec"]
+        if v in (3, 4):  # four-step variants have outer passes
+            pb["outer_fwd"] += kb["outer_fwd"]
+            pb["outer_inv"] += kb["outer_inv"]
+    dom = max((k for k in per_kernel if k in pb), key=lambda k: per_kernel[k]["ms_total"])
+    dom_s = per_kernel[dom]["ms_total"] / 1e3 / per_kernel[dom]["launches"]
+    achieved = pb[dom] / dom_s / 1e9
+    step_bytes = sum(pb[k] * per_kernel[k]["launches"] for k in per_kernel if k in pb) / a.steps
+    value = npts / t_step / 1e6
+    out = {
+        "metric": "Mpts/s per RK4 step", "value": value, "unit": "Mpts/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": t_step * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "forced 2D ensemble: 64 x 256^2 + 32 x 512^2 + 16 x 1024^2 + "
+                               "4 x 2048^2 fp32, band 4-6 constant power 1e-3, drag 0.1, "
+                               "CFL 0.5, device-resident loop, one launch per phase",
+                   "members": len(cfgs), "points": npts,
+                   "variants": {str(c.n): v for c, v in zip(cfgs, tuned["variants"])},
+                   "wg_per_cu": tuned["wg_per_cu"]},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": pb[dom], "avg_launch_us": dom_s * 1e6},
+        "per_kernel": per_kernel, "step_hbm_bytes_algorithmic": step_bytes,
+        "step_hbm_GBs": step_bytes / t_step / 1e9, "graph_ms_per_step": t_graph * 1e3,
+        "autotune_s": tune_s, "autotune": tuned["timings"], "finite": finite,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = ensemble_cpu_baseline(cfgs, states, outs)
+    ens.close()
+    print(json.dumps(out), flush=True)
+    return 0 if finite else 1
+
+
+def ensemble_cpu_baseline(cfgs, states, outs) -> dict:
+    """The CPU oracle on one IF-RK4 step of the first member of each size, scaled by the member
+    counts to one ensemble step."""
+    from oracle import numpy_oracle as O
+
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+                or (os.cpu_count() or 1))
+    seen: dict[int, int] = {}
+    for m, c in enumerate(cfgs):
+        seen.setdefault(c.n, m)
+    total = 0.0
+    per = {}
+    for n, m in seen.items():
+        s = O.OracleSolver(cfgs[m], workers=cores)
+        s.set_state(states[m])
+        s.step(outs[m]["dt"][0])
+        t0 = time.perf_counter()
+        s.step(outs[m]["dt"][0])
+        dt_s = time.perf_counter() - t0
+        count = sum(1 for c in cfgs if c.n == n)
+        per[str(n)] = dt_s
+        total += dt_s * count
+    npts = sum(c.n * c.n for c in cfgs)
+    return {"value": npts / total / 1e6, "unit": "Mpts/s", "cores": cores, "kind": "port",
+            "sample": "one timed IF-RK4 step (after one untimed) of the first member of each "
+                      "size with the oracle (scipy.fft fp32, workers=%d), times the member "
+                      "counts: %s s per member step" % (cores, per)}
+
+
 def main(argv: list[str] | None = None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -131,6 +246,8 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-roofline", action="store_true", help="skip rocFFT/copy ceilings")
+    ap.add_argument("--workload", choices=("single", "ensemble"), default="single",
+                    help="single: one 8192^2 flow (default); ensemble: the forced ensemble")
     a = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -138,6 +255,9 @@ def main(argv: list[str] | None = None) -> int:
               "MI355X (multi-GPU sharding is a design note, not a run); use --gpus 1",
               file=sys.stderr)
         return 2
+
+    if a.workload == "ensemble":
+        return ensemble_bench(a)
 
     import torch
 
