@@ -206,6 +206,74 @@ int psx_ens_run_peek(psx_ens* e, int lag, psx_run_state* out);
 int psx_ens_run_poll(psx_ens* e, psx_run_state* out, double* rec, double* steps, void* stream);
 int psx_ens_run_end(psx_ens* e, void* stream);
 
+/* ---- large-eddy simulation support (no counterpart in the reference) ------------------
+ * psx_set_source: a 3D context adds src (state layout, or null) to the nonlinear term of every
+ * stage; the LES adds the Leray-projected divergence of its subgrid stress this way. */
+int psx_set_source(psx_ctx* c, void* src);
+
+/* In-place unnormalised 3D complex FFT (complex64) of nbatch contiguous L^3 cubes
+ * (L = 64, 128, 256; z fastest); inverse = 1 for exp(+i k.x). */
+int psx_fft3(void* data, int L, int nbatch, int inverse, void* stream);
+
+/* Spectral and pointwise LES kernels (fp32 complex unless stated); `which` selects the
+ * operation (PSX_LES_*), the fields of psx_les_args are interpreted as documented there. */
+typedef struct psx_les_args {
+  const void* in;   /* main input */
+  const void* in2;  /* second input */
+  void* out;        /* output */
+  long long n;      /* count (components, pairs or values) */
+  int L, L2, pitch; /* grid size, second grid size, row pitch or flag */
+  double scale;
+} psx_les_args;
+#define PSX_LES_EXPAND 0     /* state (pitch) with n components -> full L^3 spectra */
+#define PSX_LES_GRADPACK 1   /* 3 full spectra -> n = 6 pairs (u,v)(w,ux)(uy,uz)(vx,vy)(vz,wx)(wy,wz), or n = 1: (u,v)(w,0); times scale */
+#define PSX_LES_PRODUCTS 2   /* physical velocity pairs -> pairs (uu,vv)(ww,uv)(uw,vw) */
+#define PSX_LES_TRUNCATE 3   /* fine (L) spectra, n arrays, pitch = 1: split pairs -> coarse (L2) spectra, box filter k_c = scale (0: none) */
+#define PSX_LES_PAIRUP 4     /* 2n full spectra -> n pairs, times scale */
+#define PSX_LES_FEATURES 5   /* physical gradient pairs -> bf16 [L^3][32] model input, times scale */
+#define PSX_LES_SMAGORINSKY 6 /* physical gradient pairs -> stress pairs, scale = (Cs D)^2 */
+#define PSX_LES_TAU_FROM_NET 7 /* in2: fp32 [L^3][32] model output -> stress pairs, times scale */
+#define PSX_LES_TARGET 8     /* filtered product pairs, in2: velocity pairs -> fp32 [L^3][8] stress, times scale */
+#define PSX_LES_SOURCE 9     /* stress pair spectra -> projected -i k_j tau_ij, state layout (pitch) */
+#define PSX_LES_SUMSQ 10     /* in2: n fp32 values in rows of `pitch`, first L2 of each -> += sum of squares (double) */
+int psx_les_op(int which, const psx_les_args* a, void* stream);
+
+/* Convolutional subgrid-stress network: periodic 3x3x3 convolution of nb cubes of n^3 points,
+ * 32 bf16 channels in and out (point-major), weights [27][32 out][32 in] bf16, fp32
+ * accumulation on MFMA; optional bias, ReLU, and (input-gradient use) a multiply by
+ * (mask > 0).  Output bf16 y and/or fp32 yf. */
+typedef struct psx_conv_args {
+  const void* x;
+  const void* w;
+  const float* bias;
+  const void* mask;
+  void* y;
+  float* yf;
+  int n, nb, relu;
+} psx_conv_args;
+int psx_cnn_conv(const psx_conv_args* a, void* stream);
+/* dw[27][32][32] += sum_p dz[p][o] x[p + tap][i]; db[32] += sum_p dz[p][o] (n = 64). */
+int psx_cnn_wgrad(const void* x, const void* dz, float* dw, float* db, int n, int nb,
+                  void* stream);
+/* g = 2 (y - t) / (npts nch) on the first nch channels (bf16 [npts][32]); loss += mean square. */
+int psx_cnn_mse(const float* y, const float* t, int tstride, int nch, long long npts, void* g,
+                double* loss, void* stream);
+/* Adam step on n fp32 values (bias corrections c1 = 1/(1-b1^t), c2 = 1/(1-b2^t)); writes the
+ * bf16 copy wb and, for a weight tensor (is_w), the flipped-transposed copy wt used by the
+ * input gradient: wt[26 - tap][i][o] = w[tap][o][i]. */
+typedef struct psx_adam_args {
+  float* w;
+  float* m;
+  float* v;
+  const float* g;
+  void* wb;
+  void* wt;
+  long long n;
+  double lr, b1, b2, eps, c1, c2;
+  int is_w;
+} psx_adam_args;
+int psx_cnn_adam(const psx_adam_args* a, void* stream);
+
 /* Message of the last failure on this thread. */
 const char* psx_last_error(void);
 

@@ -143,6 +143,9 @@ class OracleSolver:
         self.state = np.zeros(self.state_shape(), self.cdt)
         self.t = 0.0
         self.nstep = 0
+        # 3D only, no reference counterpart: a source held fixed over a step and added to the
+        # nonlinear term of every stage (the LES subgrid-stress divergence); state layout
+        self.source: np.ndarray | None = None
 
     # -- shared API (base.py) --------------------------------------------------------
     def state_shape(self) -> tuple[int, ...]:
@@ -235,6 +238,8 @@ class OracleSolver:
             nl = -1j * np.stack([k[0] * t11 + k[1] * t12 + k[2] * t13,
                                  k[0] * t12 + k[1] * t22 + k[2] * t23,
                                  k[0] * t13 + k[1] * t23 + k[2] * t33])
+        if self.source is not None:
+            nl = nl + self.source
         nl = nl * back_shift * self.mask
         div = (k[0] * nl[0] + k[1] * nl[1] + k[2] * nl[2]) * self.inv_k2
         nl = np.stack([nl[i] - k[i] * div for i in range(3)])

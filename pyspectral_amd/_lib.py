@@ -41,6 +41,28 @@ class RunState(ctypes.Structure):
                 ("done", c_int)]
 
 
+class LesArgs(ctypes.Structure):
+    _fields_ = [("inp", c_void_p), ("in2", c_void_p), ("out", c_void_p), ("n", c_longlong),
+                ("L", c_int), ("L2", c_int), ("pitch", c_int), ("scale", c_double)]
+
+
+class ConvArgs(ctypes.Structure):
+    _fields_ = [("x", c_void_p), ("w", c_void_p), ("bias", c_void_p), ("mask", c_void_p),
+                ("y", c_void_p), ("yf", c_void_p), ("n", c_int), ("nb", c_int), ("relu", c_int)]
+
+
+class AdamArgs(ctypes.Structure):
+    _fields_ = [("w", c_void_p), ("m", c_void_p), ("v", c_void_p), ("g", c_void_p),
+                ("wb", c_void_p), ("wt", c_void_p), ("n", c_longlong), ("lr", c_double),
+                ("b1", c_double), ("b2", c_double), ("eps", c_double), ("c1", c_double),
+                ("c2", c_double), ("is_w", c_int)]
+
+
+LES_OPS = {"expand": 0, "gradpack": 1, "products": 2, "truncate": 3, "pairup": 4,
+           "features": 5, "smagorinsky": 6, "tau_from_net": 7, "target": 8, "source": 9,
+           "sumsq": 10}  # PSX_LES_*
+
+
 # name -> (restype, argtypes); the exported C ABI
 SIGNATURES = {
     "psx_query_layout": (c_int, [POINTER(FlowParams), POINTER(Layout)]),
@@ -79,6 +101,14 @@ SIGNATURES = {
     "psx_ens_run_peek": (c_int, [c_void_p, c_int, POINTER(RunState)]),
     "psx_ens_run_poll": (c_int, [c_void_p, POINTER(RunState), c_void_p, c_void_p, c_void_p]),
     "psx_ens_run_end": (c_int, [c_void_p, c_void_p]),
+    "psx_set_source": (c_int, [c_void_p, c_void_p]),
+    "psx_fft3": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
+    "psx_les_op": (c_int, [c_int, POINTER(LesArgs), c_void_p]),
+    "psx_cnn_conv": (c_int, [POINTER(ConvArgs), c_void_p]),
+    "psx_cnn_wgrad": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    "psx_cnn_mse": (c_int, [c_void_p, c_void_p, c_int, c_int, c_longlong, c_void_p, c_void_p,
+                            c_void_p]),
+    "psx_cnn_adam": (c_int, [POINTER(AdamArgs), c_void_p]),
     "psx_last_error": (c_char_p, []),
     "psx_device_info": (c_int, [c_char_p, c_int]),
 }

@@ -76,6 +76,7 @@ class HipSolver(SpectralSolver):
     variant = "fused"
     default_form = "basdevant"
     forms = ("basdevant", "rotational")
+    device_loop = True  # run_device: the time loop of simulate.run on the GPU
 
     def __init__(self, cfg: Config, graph: bool = False) -> None:
         super().__init__(cfg)

@@ -693,6 +693,22 @@ int psx_spectrum(psx_ctx* c, const void* state, int pitch, double* out, int nshe
   return 0;
 }
 
+int psx_set_source(psx_ctx* c, void* src) {
+  if (!c) return fail("psx_set_source: null context");
+  if (c->p.dim != 3) return fail("psx_set_source: 3D flows only");
+  if (c->fp64) c->f64.src = static_cast<const cx<double>*>(src);
+  else c->f32.src = static_cast<const cx<float>*>(src);
+  // the flow descriptor is a kernel argument: captured graphs hold the old one
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->rgexec) (void)hipGraphExecDestroy(c->rgexec);
+  if (c->rgraph) (void)hipGraphDestroy(c->rgraph);
+  c->gexec = c->rgexec = nullptr;
+  c->graph = c->rgraph = nullptr;
+  c->run_chunk = 0;
+  return 0;
+}
+
 // ------------------------------------------------------------------ device controller
 int psx_run_begin(psx_ctx* c, const psx_run_params* rp, void* stream) {
   if (!c || !rp) return fail("psx_run_begin: null argument");

@@ -34,6 +34,8 @@ struct Flow {
   const cx<T>* twp;  // per-pass twiddle tables of all plans (FftPlan::twoff)
   const double* prm; // 2*nh table entries followed by PrmLayout scalars
   unsigned long long* stats;  // CFL maxima max|u_i| as FP64 bit patterns, 3 entries
+  const cx<T>* src;  // 3D: optional source added to the nonlinear term of every stage, in the
+                     // state layout (null: none); e.g. a subgrid-stress divergence
   long long fstride; // complex elements per work field (nrows * ldw)
   long long sstride; // complex elements per state component (3D)
   int n, nh, jm, ldw;

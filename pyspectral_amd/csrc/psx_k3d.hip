@@ -319,6 +319,11 @@ __global__ void __launch_bounds__(512, 1) k_spec3d(Flow<T> f, int stage) {
         } else {
           const T fx = (T)kx, fy = (T)ky, fz = (T)kz;
           const T ik2 = k2 > 0 ? (T)1 / (T)k2 : (T)0;
+          if (f.src) {
+            a0[i] = cadd(a0[i], f.src[si]);
+            a1[i] = cadd(a1[i], f.src[sst + si]);
+            a2[i] = cadd(a2[i], f.src[2 * sst + si]);
+          }
           const cx<T> d = cscl(ik2, cadd(cadd(cscl(fx, a0[i]), cscl(fy, a1[i])), cscl(fz, a2[i])));
           const cx<T> n0 = csub(a0[i], cscl(fx, d));
           const cx<T> n1 = csub(a1[i], cscl(fy, d));

@@ -359,6 +359,12 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
         const int k2 = kx * kx + ky * ky + kz * kz;
         const T fx = (T)kx, fy = (T)ky, fz = (T)kz;
         const T ik2 = k2 > 0 ? __builtin_amdgcn_rcpf((T)k2) : (T)0;
+        if (f.src) {  // uniform: an additive source (state layout, zero outside the mask)
+          const BufC32 sb(const_cast<cx<T>*>(f.src), (unsigned long long)3 * SST * 8);
+          a0[i] = cadd(a0[i], sb.load(so(i, 0)));
+          a1[i] = cadd(a1[i], sb.load(so(i, 1)));
+          a2[i] = cadd(a2[i], sb.load(so(i, 2)));
+        }
         const cx<T> d = cscl(ik2, cadd(cadd(cscl(fx, a0[i]), cscl(fy, a1[i])), cscl(fz, a2[i])));
         const cx<T> n0 = csub(a0[i], cscl(fx, d));
         const cx<T> n1 = csub(a1[i], cscl(fy, d));

@@ -1,0 +1,288 @@
+"""Large-eddy simulation with a learned subgrid-stress closure (3D, fp32, 64^3 coarse grid).
+
+No counterpart in the reference.  Pieces, all on libpsx kernels:
+- ``FilterPipeline``: a fine DNS state (the hip solver's own 256^3 state) -> the box-filtered
+  (transfer function prod_i sin(k_i D/2)/(k_i D/2), D = pi/k_c), truncated velocity on the
+  coarse grid, the model input (filtered velocity gradient, 9 channels) and the subgrid-stress
+  target tau_ij = filt(u_i u_j) - filt(u_i) filt(u_j) (6 channels).
+- ``SgsNet``: six periodic 3x3x3 convolutions with 32 channels (ReLU between them), bf16
+  weights and activations, fp32 accumulation on MFMA; hand-written forward, backward
+  (input and weight gradients) and Adam kernels.
+- ``LESSolver``: the fused 64^3 IF-RK4 step plus the Leray-projected divergence of a subgrid
+  stress (none, Smagorinsky, or the network), added to the nonlinear term of every stage.
+  The stress is evaluated from the state at the start of a step (every ``sgs_every`` steps)
+  and held over the step.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+from typing import Any
+
+import numpy as np
+
+from pyspectral_amd import _lib
+from pyspectral_amd.backends.base import BackendUnavailable
+from pyspectral_amd.backends.hip_fused import HipSolver
+from pyspectral_amd.config import Config
+
+C = 32          # channels of every network tensor
+TAU_C = 8       # channels of a stored stress target (6 used)
+TAPS = 27
+NLAYER = 6
+IN_CH, OUT_CH = 9, 6
+WSIZE = TAPS * C * C
+
+
+def _p(t: Any) -> ctypes.c_void_p:
+    return ctypes.c_void_p(t.data_ptr() if t is not None else 0)
+
+
+def fft3(t: Any, L: int, nbatch: int, inverse: bool) -> None:
+    """In-place unnormalised 3D complex FFT of nbatch L^3 cubes (complex64)."""
+    lib = _lib.load()
+    _lib.check(lib.psx_fft3(_p(t), L, nbatch, int(inverse), _lib.stream_ptr()))
+
+
+def les_op(name: str, inp: Any, out: Any, *, n: int = 0, L: int = 0, L2: int = 0,
+           pitch: int = 0, scale: float = 1.0, in2: Any = None) -> None:
+    """One of the LES support kernels (include/psx.h, PSX_LES_*)."""
+    lib = _lib.load()
+    a = _lib.LesArgs(inp=_p(inp).value, in2=_p(in2).value, out=_p(out).value, n=n, L=L, L2=L2,
+                     pitch=pitch, scale=scale)
+    _lib.check(lib.psx_les_op(_lib.LES_OPS[name], ctypes.byref(a), _lib.stream_ptr()))
+
+
+def rms(t: Any, width: int, count: int) -> float:
+    """RMS over the first ``count`` of every ``width`` fp32 values of t (FP64 sum)."""
+    import torch
+
+    acc = torch.zeros(1, dtype=torch.float64, device=t.device)
+    les_op("sumsq", None, acc, n=t.numel(), L=4, L2=count, pitch=width, in2=t)
+    return math.sqrt(float(acc.item()) / (t.numel() // width * count))
+
+
+def conv(x: Any, w: Any, *, n: int, nb: int, bias: Any = None, mask: Any = None, y: Any = None,
+         yf: Any = None, relu: bool = False) -> None:
+    lib = _lib.load()
+    a = _lib.ConvArgs(x=_p(x).value, w=_p(w).value, bias=_p(bias).value, mask=_p(mask).value,
+                      y=_p(y).value, yf=_p(yf).value, n=n, nb=nb, relu=int(relu))
+    _lib.check(lib.psx_cnn_conv(ctypes.byref(a), _lib.stream_ptr()))
+
+
+# --------------------------------------------------------------------------------- network
+class SgsNet:
+    """The subgrid-stress network and its Adam state (fp32 master weights on the device)."""
+
+    def __init__(self, n: int = 64, seed: int = 0, lr: float = 1e-3,
+                 betas: tuple[float, float] = (0.9, 0.999), eps: float = 1e-8) -> None:
+        import torch
+
+        self.torch = torch
+        self.n = n
+        self.lr, self.b1, self.b2, self.eps = lr, betas[0], betas[1], eps
+        rng = np.random.default_rng(seed)
+        w = np.zeros((NLAYER, TAPS, C, C), dtype=np.float32)
+        for layer in range(NLAYER):
+            cin = IN_CH if layer == 0 else C
+            cout = OUT_CH if layer == NLAYER - 1 else C
+            bound = math.sqrt((6.0 if layer < NLAYER - 1 else 3.0) / (TAPS * cin))
+            w[layer, :, :cout, :cin] = rng.uniform(-bound, bound, (TAPS, cout, cin))
+        dev = "cuda"
+        f32, bf16 = torch.float32, torch.bfloat16
+        self.w = torch.from_numpy(w.reshape(NLAYER, WSIZE)).to(dev)
+        self.b = torch.zeros((NLAYER, C), dtype=f32, device=dev)
+        self.mw, self.vw = torch.zeros_like(self.w), torch.zeros_like(self.w)
+        self.mb, self.vb = torch.zeros_like(self.b), torch.zeros_like(self.b)
+        self.gw, self.gb = torch.zeros_like(self.w), torch.zeros_like(self.b)
+        self.wb = torch.zeros((NLAYER, WSIZE), dtype=bf16, device=dev)
+        self.wt = torch.zeros((NLAYER, WSIZE), dtype=bf16, device=dev)
+        self.t = 0
+        self._acts: dict[int, list[Any]] = {}
+        self._adam(lr=0.0)  # bf16 copies of the initial weights (zero step)
+
+    def _buffers(self, nb: int) -> list[Any]:
+        torch = self.torch
+        if nb not in self._acts:
+            P = nb * self.n ** 3
+            acts = [torch.zeros((P, C), dtype=torch.bfloat16, device="cuda")
+                    for _ in range(NLAYER - 1)]
+            out = torch.zeros((P, C), dtype=torch.float32, device="cuda")
+            g = [torch.zeros((P, C), dtype=torch.bfloat16, device="cuda") for _ in range(2)]
+            self._acts[nb] = acts + [out] + g
+        return self._acts[nb]
+
+    def forward(self, X: Any, nb: int) -> Any:
+        """Network output (fp32 [nb n^3][32], channels 0..5 meaningful) for input X."""
+        buf = self._buffers(nb)
+        x = X
+        for layer in range(NLAYER):
+            last = layer == NLAYER - 1
+            conv(x, self.wb[layer], n=self.n, nb=nb, bias=self.b[layer],
+                 y=None if last else buf[layer], yf=buf[NLAYER - 1] if last else None,
+                 relu=not last)
+            if not last:
+                x = buf[layer]
+        return buf[NLAYER - 1]
+
+    def eval_loss(self, X: Any, T: Any, nb: int) -> float:
+        """Mean-square loss of the current network on (X, T) (no update)."""
+        lib = _lib.load()
+        loss = self.torch.zeros(1, dtype=self.torch.float64, device="cuda")
+        y = self.forward(X, nb)
+        g = self._buffers(nb)[NLAYER]
+        _lib.check(lib.psx_cnn_mse(_p(y), _p(T), TAU_C, OUT_CH, nb * self.n ** 3, _p(g),
+                                   _p(loss), _lib.stream_ptr()))
+        return float(loss.item())
+
+    def train_step(self, X: Any, T: Any, nb: int, loss: Any) -> None:
+        """Forward, mean-square loss against T (fp32 [nb n^3][8]; accumulated into the device
+        scalar ``loss``), backward and one Adam step."""
+        self.gradients(X, T, nb, loss)
+        self._adam()
+
+    def gradients(self, X: Any, T: Any, nb: int, loss: Any) -> None:
+        """Forward, loss and backward: weight and bias gradients into gw, gb."""
+        lib = _lib.load()
+        st = _lib.stream_ptr()
+        buf = self._buffers(nb)
+        y = self.forward(X, nb)
+        P = nb * self.n ** 3
+        g = buf[NLAYER]
+        _lib.check(lib.psx_cnn_mse(_p(y), _p(T), TAU_C, OUT_CH, P, _p(g), _p(loss), st))
+        self.gw.zero_()
+        self.gb.zero_()
+        dz, spare = g, buf[NLAYER + 1]
+        for layer in range(NLAYER - 1, -1, -1):
+            x_in = X if layer == 0 else buf[layer - 1]
+            _lib.check(lib.psx_cnn_wgrad(_p(x_in), _p(dz), _p(self.gw[layer]),
+                                         _p(self.gb[layer]), self.n, nb, st))
+            if layer > 0:
+                conv(dz, self.wt[layer], n=self.n, nb=nb, mask=buf[layer - 1], y=spare)
+                dz, spare = spare, dz
+
+    def _adam(self, lr: float | None = None) -> None:
+        lib = _lib.load()
+        st = _lib.stream_ptr()
+        lr = self.lr if lr is None else lr
+        if lr > 0:
+            self.t += 1
+        t = max(self.t, 1)
+        c1 = 1.0 / (1.0 - self.b1 ** t)
+        c2 = 1.0 / (1.0 - self.b2 ** t)
+        for layer in range(NLAYER):
+            a = _lib.AdamArgs(w=_p(self.w[layer]).value, m=_p(self.mw[layer]).value,
+                              v=_p(self.vw[layer]).value, g=_p(self.gw[layer]).value,
+                              wb=_p(self.wb[layer]).value, wt=_p(self.wt[layer]).value,
+                              n=WSIZE, lr=lr, b1=self.b1, b2=self.b2, eps=self.eps, c1=c1,
+                              c2=c2, is_w=1)
+            _lib.check(lib.psx_cnn_adam(ctypes.byref(a), st))
+            a = _lib.AdamArgs(w=_p(self.b[layer]).value, m=_p(self.mb[layer]).value,
+                              v=_p(self.vb[layer]).value, g=_p(self.gb[layer]).value,
+                              wb=None, wt=None, n=C, lr=lr, b1=self.b1, b2=self.b2,
+                              eps=self.eps, c1=c1, c2=c2, is_w=0)
+            _lib.check(lib.psx_cnn_adam(ctypes.byref(a), st))
+
+    def state(self) -> dict[str, np.ndarray]:
+        return {"w": self.w.cpu().numpy(), "b": self.b.cpu().numpy()}
+
+
+# ------------------------------------------------------------------------ filtered DNS
+class FilterPipeline:
+    """Fine DNS state -> filtered coarse velocity spectra, model input and stress target."""
+
+    def __init__(self, Lf: int, Lc: int = 64, kc: float = 21.0) -> None:
+        import torch
+
+        self.torch = torch
+        self.Lf, self.Lc, self.kc = Lf, Lc, kc
+        c64 = torch.complex64
+        Lf3, Lc3 = Lf ** 3, Lc ** 3
+        self.full = torch.zeros(3 * Lf3, dtype=c64, device="cuda")
+        self.vel = torch.zeros(2 * Lf3, dtype=c64, device="cuda")
+        self.prod = torch.zeros(3 * Lf3, dtype=c64, device="cuda")
+        self.fprod_s = torch.zeros(6 * Lc3, dtype=c64, device="cuda")
+        self.ubar = torch.zeros(3 * Lc3, dtype=c64, device="cuda")
+        self.grad = torch.zeros(6 * Lc3, dtype=c64, device="cuda")
+        self.fprod = torch.zeros(3 * Lc3, dtype=c64, device="cuda")
+
+    def __call__(self, state_rows: Any, pitch: int, X: Any, T: Any, g_scale: float = 1.0,
+                 tau_scale: float = 1.0) -> Any:
+        """state_rows: the fine state (3, Lf, Lf, pitch) complex64; writes X (bf16 [Lc^3][32])
+        and T (fp32 [Lc^3][8]); returns the filtered coarse velocity, full spectra (3 Lc^3)."""
+        Lf, Lc, kc = self.Lf, self.Lc, self.kc
+        les_op("expand", state_rows, self.full, n=3, L=Lf, pitch=pitch)
+        les_op("gradpack", self.full, self.vel, n=1, L=Lf, scale=1.0 / Lf ** 3)
+        fft3(self.vel, Lf, 2, inverse=True)
+        les_op("products", self.vel, self.prod, L=Lf)
+        fft3(self.prod, Lf, 3, inverse=False)
+        les_op("truncate", self.prod, self.fprod_s, n=3, L=Lf, L2=Lc, pitch=1, scale=kc)
+        les_op("truncate", self.full, self.ubar, n=3, L=Lf, L2=Lc, pitch=0, scale=kc)
+        les_op("gradpack", self.ubar, self.grad, n=6, L=Lc, scale=1.0 / Lc ** 3)
+        fft3(self.grad, Lc, 6, inverse=True)
+        les_op("pairup", self.fprod_s, self.fprod, n=3, L=Lc, scale=1.0 / Lc ** 3)
+        fft3(self.fprod, Lc, 3, inverse=True)
+        les_op("target", self.fprod, T, L=Lc, scale=tau_scale, in2=self.grad)
+        les_op("features", self.grad, X, L=Lc, scale=g_scale)
+        return self.ubar
+
+    def half_state(self, full: Any) -> Any:
+        """NumPy-layout (rfftn) view of full coarse spectra: (3, Lc, Lc, Lc/2+1)."""
+        Lc = self.Lc
+        return full.view(3, Lc, Lc, Lc)[..., : Lc // 2 + 1]
+
+
+# ------------------------------------------------------------------------------ LES solver
+class LESSolver(HipSolver):
+    """Coarse 3D fp32 flow with a subgrid stress: model in {"none", "smagorinsky", "net"}."""
+
+    variant = "les"
+    device_loop = False  # the stress is computed on the host's schedule
+
+    def __init__(self, cfg: Config, model: str = "none", *, net: SgsNet | None = None,
+                 cs: float = 0.17, kc: float = 21.0, g_scale: float = 1.0,
+                 tau_scale: float = 1.0, sgs_every: int = 1) -> None:
+        if cfg.dim != 3 or cfg.dtype != "float32":
+            raise BackendUnavailable("LES: 3D fp32 flows")
+        if model not in ("none", "smagorinsky", "net"):
+            raise ValueError(f"unknown subgrid model {model!r}")
+        if model == "net" and net is None:
+            raise ValueError("the network model needs a trained SgsNet")
+        super().__init__(cfg)
+        torch = self.torch
+        L = cfg.n
+        self.model, self.net, self.cs, self.kc = model, net, cs, kc
+        self.g_scale, self.tau_scale, self.sgs_every = g_scale, tau_scale, max(1, sgs_every)
+        self.src = torch.zeros_like(self.buf.u0_rows)
+        self.full = torch.zeros(3 * L ** 3, dtype=torch.complex64, device="cuda")
+        self.grad = torch.zeros(6 * L ** 3, dtype=torch.complex64, device="cuda")
+        self.tau = torch.zeros(3 * L ** 3, dtype=torch.complex64, device="cuda")
+        self.X = torch.zeros((L ** 3, C), dtype=torch.bfloat16, device="cuda")
+        if model != "none":
+            _lib.check(self.lib.psx_set_source(self.ctx, _p(self.src)))
+        self._since = 0
+
+    def update_source(self) -> None:
+        """Subgrid-stress divergence of the current state into the source buffer."""
+        L = self.n
+        les_op("expand", self.buf.u0_rows, self.full, n=3, L=L, pitch=self.buf.pitch)
+        les_op("gradpack", self.full, self.grad, n=6, L=L, scale=1.0 / L ** 3)
+        fft3(self.grad, L, 6, inverse=True)
+        if self.model == "smagorinsky":
+            delta = math.pi / self.kc
+            les_op("smagorinsky", self.grad, self.tau, L=L, scale=(self.cs * delta) ** 2)
+        else:
+            les_op("features", self.grad, self.X, L=L, scale=self.g_scale)
+            y = self.net.forward(self.X, 1)  # type: ignore[union-attr]
+            les_op("tau_from_net", None, self.tau, L=L, scale=self.tau_scale, in2=y)
+        fft3(self.tau, L, 3, inverse=False)
+        les_op("source", self.tau, self.src, L=L, pitch=self.buf.pitch)
+
+    def _step(self, dt: float) -> None:
+        if self.model != "none":
+            if self._since % self.sgs_every == 0:
+                self.update_source()
+            self._since += 1
+        super()._step(dt)

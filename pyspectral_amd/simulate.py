@@ -7,8 +7,8 @@ or callback time; with ``target_power`` the band growth rate is reset every ``cf
 steps to A = P / (2 E_band); (t, E, Z) is recorded at the start, at every diagnostics time
 and at the end.
 
-Two drivers implement them.  Solvers with a device controller (``run_device``: the hip
-variants) run the whole loop on the GPU in captured chunks, and the host only collects the
+Two drivers implement them.  Solvers with a device controller (``device_loop`` true: the
+hip variants) run the whole loop on the GPU in captured chunks, and the host only collects the
 history; that is the default when no callback or wall-clock limit asks for host control
 between steps.  Every other case runs the host loop below, one ``solver.step`` at a time.
 """
@@ -71,7 +71,8 @@ def run(solver: Any, t_end: float, *, dt: float | None = None, cfl: float = 0.5,
     """Integrate ``solver`` to ``t_end``; ``device`` forces (True) or forbids (False) the
     device controller, None picks it whenever the run allows it."""
     res = result if result is not None else RunResult()
-    can_device = hasattr(solver, "run_device") and callback is None and wall_limit is None
+    can_device = (getattr(solver, "device_loop", False) and callback is None
+                  and wall_limit is None)
     if device and not can_device:
         raise ValueError("the device controller needs a hip solver, no callback and no "
                          "wall-clock limit")
