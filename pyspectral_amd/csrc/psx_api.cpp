@@ -422,8 +422,9 @@ int enqueue_step(psx_ctx* c, hipStream_t st, int drv = 0) {
       PSX_HIP(launch_ypass3d<T>(f, c->yI, true, f.rot ? 6 : 3, st));
     }
   }
+  PSX_MARK(PSX_KIND_CTRL);
+  PSX_HIP(launch_hermitian<T>(f, st));
   if (drv) {
-    PSX_MARK(PSX_KIND_CTRL);
     PSX_HIP(launch_drv(1, c->dmem, 1, st));
     PSX_HIP(launch_drv(2, c->dmem, 1, st));
   }

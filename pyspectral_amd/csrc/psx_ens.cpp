@@ -174,8 +174,13 @@ int ens_step(psx_ens* e, hipStream_t st, int drv) {
       if ((r = ens_phase(e, 3, 0, st))) return r;
     }
   }
+  ENS_MARK(PSX_KIND_CTRL);
+  {
+    int nmax = 0;
+    for (const psx_flow_params& p : e->p) nmax = std::max(nmax, p.n);
+    PSX_HIP(launch_ens_hermitian(e->dflows, e->count, nmax, st));
+  }
   if (drv) {
-    ENS_MARK(PSX_KIND_CTRL);
     PSX_HIP(launch_drv(1, e->dmem, e->count, st));
     PSX_HIP(launch_drv(2, e->dmem, e->count, st));
   }

@@ -28,6 +28,8 @@ constexpr int ens_code(int n, int var) { return n * 8 + var; }
 // forward, 3 outer inverse.
 hipError_t launch_ens(int kind, int arg, const EnsFlow* flows, const EnsItem* items, int nitems,
                       int grid, size_t lds, hipStream_t st);
+// Hermitian projection of every member's ky = 0 column after a step (nmax: largest n).
+hipError_t launch_ens_hermitian(const EnsFlow* flows, int count, int nmax, hipStream_t st);
 // True if (n, var) has compiled ensemble bodies.
 bool ens_compiled(int n, int var);
 

@@ -316,6 +316,11 @@ __global__ void __launch_bounds__(512, 1) k_spec3d(Flow<T> f, int stage) {
           ua = f.u0[si];
           ub = f.u0[sst + si];
           uc = f.u0[2 * sst + si];
+          const T ik2 = k2 > 0 ? (T)1 / (T)k2 : (T)0;
+          leray<T>((T)kx, (T)ky, (T)kz, ik2, ua, ub, uc);
+          f.u0[si] = ua;
+          f.u0[sst + si] = ub;
+          f.u0[2 * sst + si] = uc;
         } else {
           const T fx = (T)kx, fy = (T)ky, fz = (T)kz;
           const T ik2 = k2 > 0 ? (T)1 / (T)k2 : (T)0;
@@ -334,6 +339,12 @@ __global__ void __launch_bounds__(512, 1) k_spec3d(Flow<T> f, int stage) {
           ub = rk_update<T>(stage, dt, (T)E1, (T)Eh, n1, f.u0 + sst + si, f.acc + sst + si);
           uc = rk_update<T>(stage, dt, (T)E1, (T)Eh, n2, f.u0 + 2 * sst + si,
                             f.acc + 2 * sst + si);
+          if (stage == 3) {  // the new state, projected (rk_update stored it unprojected)
+            leray<T>(fx, fy, fz, ik2, ua, ub, uc);
+            f.u0[si] = ua;
+            f.u0[sst + si] = ub;
+            f.u0[2 * sst + si] = uc;
+          }
         }
         ua = cscl(f.scale, ua);
         ub = cscl(f.scale, ub);

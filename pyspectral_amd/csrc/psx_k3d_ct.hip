@@ -355,6 +355,14 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
       const int kz = j0 + t;
       const cx<T> u0a = ub.load(so(i, 0)), u0b = ub.load(so(i, 1)), u0c = ub.load(so(i, 2));
       cx<T> ua = u0a, ubv = u0b, uc = u0c;
+      if constexpr (stage == kStagePre) {
+        const int k2 = kx * kx + ky * ky + kz * kz;
+        const T ik2 = k2 > 0 ? __builtin_amdgcn_rcpf((T)k2) : (T)0;
+        leray<T>((T)kx, (T)ky, (T)kz, ik2, ua, ubv, uc);
+        ub.store(so(i, 0), ua);
+        ub.store(so(i, 1), ubv);
+        ub.store(so(i, 2), uc);
+      }
       if constexpr (stage != kStagePre) {
         const int k2 = kx * kx + ky * ky + kz * kz;
         const T fx = (T)kx, fy = (T)ky, fz = (T)kz;
@@ -385,6 +393,7 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
           ab.store(so(i, 1), c1);
           ab.store(so(i, 2), c2);
         } else {
+          leray<T>(fx, fy, fz, ik2, ua, ubv, uc);
           ub.store(so(i, 0), ua);
           ub.store(so(i, 1), ubv);
           ub.store(so(i, 2), uc);

@@ -56,6 +56,32 @@ __global__ void __launch_bounds__(kEnsNT, 2)
   }
 }
 
+// Hermitian projection of every member's ky = 0 column (k_hermitian, one member per grid row).
+__global__ void ke_hermitian(const EnsFlow* __restrict__ flows) {
+  const Flow<float>& f = flows[blockIdx.y].f;
+  if (flow_inactive(f.prm, f.nh)) return;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = f.n;
+  if (r >= n) return;
+  const int rm = r ? n - r : 0;
+  if (rm < r) return;
+  cx<float>* a = f.u0 + (long long)r * f.sp;
+  if (rm == r) {
+    a->y = 0.f;
+    return;
+  }
+  cx<float>* b = f.u0 + (long long)rm * f.sp;
+  const cx<float> ua = *a, ub = *b;
+  const cx<float> h = mkc(0.5f * (ua.x + ub.x), 0.5f * (ua.y - ub.y));
+  *a = h;
+  *b = conjg(h);
+}
+
+hipError_t launch_ens_hermitian(const EnsFlow* flows, int count, int nmax, hipStream_t st) {
+  hipLaunchKernelGGL(ke_hermitian, dim3((nmax + 255) / 256, count), dim3(256), 0, st, flows);
+  return hipGetLastError();
+}
+
 bool ens_compiled(int n, int var) {
   switch (ens_code(n, var)) {
 #define PSX_X(NV, VV) case ens_code(NV, VV):

@@ -66,6 +66,9 @@ hipError_t launch_ct3d(const Flow<float>& f, int which, int arg, hipStream_t st)
 hipError_t launch_drv(int which, const DrvMember* mem, int count, hipStream_t st);
 
 // Utility / diagnostics kernels (psx_kutil.hip).
+// Hermitian projection of the state's self-mirrored plane after a step (k_hermitian).
+template <typename T>
+hipError_t launch_hermitian(const Flow<T>& f, hipStream_t st);
 hipError_t launch_params(double* prm, int nh, double nu, double dt, double alpha, double amp,
                          double klo2, double khi2, double ka2, hipStream_t st);
 hipError_t launch_fill_u32(unsigned* p, unsigned v, int n, hipStream_t st);

@@ -39,6 +39,55 @@ hipError_t launch_fill_f64(double* p, double v, int n, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Hermitian projection of the self-mirrored half-axis plane (3D: kz = 0 plane of every
+// component; 2D: the ky = 0 column): u(k) <- (u(k) + conj u(-k)) / 2 and u(-k) <- conj of
+// that, by the one thread that owns the pair.  A real field has no anti-Hermitian part
+// there; the transforms drop it (a c2r uses the real part of that plane), so it takes no part
+// in the nonlinear term and only the linear factors act on it.  Round-off of the forward
+// transforms feeds it a little every step, and inside a forcing band whose growth rate
+// exceeds the viscous decay it would grow without bound; the projection removes it after
+// every step.  The state is left unchanged up to that round-off.
+template <typename T>
+__global__ void k_hermitian(cx<T>* __restrict__ u0, int n, int sp, int dim, long long sstride,
+                            const double* __restrict__ prm) {
+  if (flow_inactive(prm, n / 2 + 1)) return;
+  const long long rows = dim == 2 ? (long long)n : (long long)n * n;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ncomp = dim == 2 ? 1 : 3;
+  if (idx >= ncomp * rows) return;
+  const int c = (int)(idx / rows);
+  const long long r = idx - c * rows;
+  long long rm;
+  if (dim == 2) {
+    rm = r ? n - r : 0;
+  } else {
+    const int x = (int)(r / n), y = (int)(r % n);
+    rm = (long long)(x ? n - x : 0) * n + (y ? n - y : 0);
+  }
+  if (rm < r) return;  // the mirror's owner handles the pair
+  cx<T>* a = u0 + c * sstride + r * sp;
+  if (rm == r) {
+    a->y = (T)0;
+    return;
+  }
+  cx<T>* b = u0 + c * sstride + rm * sp;
+  const cx<T> ua = *a, ub = *b;
+  const cx<T> h = mkc((T)0.5 * (ua.x + ub.x), (T)0.5 * (ua.y - ub.y));
+  *a = h;
+  *b = conjg(h);
+}
+
+template <typename T>
+hipError_t launch_hermitian(const Flow<T>& f, hipStream_t st) {
+  const long long rows = f.dim == 2 ? (long long)f.n : (long long)f.n * f.n;
+  const long long nt = (f.dim == 2 ? 1 : 3) * rows;
+  hipLaunchKernelGGL(k_hermitian<T>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, st,
+                     f.u0, f.n, f.sp, f.dim, f.sstride, f.prm);
+  return hipGetLastError();
+}
+template hipError_t launch_hermitian<float>(const Flow<float>&, hipStream_t);
+template hipError_t launch_hermitian<double>(const Flow<double>&, hipStream_t);
+
 // Mode idx of a state component in NumPy layout -> integer wavenumbers and Hermitian weight.
 __device__ __forceinline__ void decode_mode(long long idx, int n, int nh, int dim, int& kx,
                                             int& ky, int& kz) {

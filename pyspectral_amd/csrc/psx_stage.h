@@ -152,6 +152,19 @@ __device__ __forceinline__ void block_max_to(double (&m)[NV],
   __syncthreads();
 }
 
+// Leray projection of one mode of a 3D state: u <- u - k (k.u) / k^2 (ik2 = 1/k^2, 0 at k = 0).
+// Applied to the new state after every step and to a loaded state: the nonlinear term is
+// projected in every stage, but the linear factors would keep (and, inside a forcing band,
+// amplify) any compressive part the state carries, such as round-off of the projection itself,
+// and the product forms conserve energy only for divergence-free velocities.
+template <typename T>
+__device__ __forceinline__ void leray(T fx, T fy, T fz, T ik2, cx<T>& a, cx<T>& b, cx<T>& c) {
+  const cx<T> d = cscl(ik2, cadd(cadd(cscl(fx, a), cscl(fy, b)), cscl(fz, c)));
+  a = csub(a, cscl(fx, d));
+  b = csub(b, cscl(fy, d));
+  c = csub(c, cscl(fz, d));
+}
+
 __device__ __forceinline__ double abs_or_inf(double a) {
   return __builtin_isnan(a) ? __builtin_inf() : fabs(a);
 }
