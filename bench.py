@@ -13,7 +13,7 @@ bandwidth (the reference's definition, src/pyspectral/benchmark.py:437-443); a r
 object for the dominant kernel from HIP events around every launch of the timed region; and
 the CPU oracle (scipy.fft) timed on a bounded sample on this host.
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--no-cpu] [--workload ensemble]
+    python bench.py [--steps K] [--warmup W] [--n N] [--no-cpu] [--workload ensemble|tgv3d]
 """
 
 from __future__ import annotations
@@ -35,6 +35,7 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0        # MI355X HBM3E
 REF_MPTS = 547.0             # reference 2D 8192^2 fp32 (cupy-cuda-c/fused-inplace, RTX 3070 L)
+REF_MPTS_3D = 310.0          # reference 3D 384^3 fp32 (cupy-cuda-c/fused-inplace-lean, same GPU)
 
 
 def kernel_bytes(n: int, jm: int, csz: int) -> dict[str, float]:
@@ -49,6 +50,24 @@ def kernel_bytes(n: int, jm: int, csz: int) -> dict[str, float]:
     m = float(2 * jm - 1) * jm * csz
     return {"line": 4 * field, "outer_fwd": 4 * field, "outer_inv": 4 * field,
             "spec": 4 * field + 2.75 * m}
+
+
+def kernel_bytes_3d(n: int, jm: int, ldw: int, rot: bool, csz: int) -> dict[str, float]:
+    """Compulsory HBM bytes per launch of each 3D kernel kind.
+
+    A work field holds the n^2 (x, y) rows of ldw >= jm retained kz columns.  Per stage:
+    the z-line kernel reads the 3 (rotational: 6) velocity/vorticity fields and writes the 5
+    (3) product fields; the forward y pass reads and writes the 5 (3) product fields; the
+    spectral (x) kernel reads them, writes the 3 (6) next-stage input fields and reads/writes
+    the 3-component state and accumulator on the (2 jm - 1)^2 jm retained modes (2.75
+    accesses per mode on average over a step, as in 2D); the inverse y pass reads and writes
+    the 3 (6) input fields.
+    """
+    field = float(n) * n * ldw * csz
+    m = 3.0 * (2 * jm - 1) ** 2 * jm * csz
+    nin, nout = (6, 3) if rot else (3, 5)
+    return {"line": (nin + nout) * field, "mid_fwd": 2 * nout * field,
+            "spec": (nout + nin) * field + 2.75 * m, "mid_inv": 2 * nin * field}
 
 
 def copy_bandwidth(torch, nbytes: int = 1 << 30, repeats: int = 10) -> float:
@@ -71,15 +90,20 @@ def copy_bandwidth(torch, nbytes: int = 1 << 30, repeats: int = 10) -> float:
     return 2 * nbytes / float(np.median(ts)) / 1e9
 
 
-def rocfft_stage(torch, n: int, n_inv: int = 2, n_fwd: int = 2, repeats: int = 20) -> float:
-    """Standalone rocFFT time (s) of one stage's batched C2R(n_inv) + R2C(n_fwd), fp32."""
+def rocfft_stage(torch, n: int, n_inv: int = 2, n_fwd: int = 2, repeats: int = 20,
+                 dim: int = 2) -> float:
+    """Standalone rocFFT time (s) of one stage's batched C2R(n_inv) + R2C(n_fwd), fp32, on
+    n^dim grids."""
     nh = n // 2 + 1
-    Q = torch.zeros((max(n_inv, n_fwd), n, nh), dtype=torch.complex64, device="cuda")
-    R = torch.zeros((max(n_inv, n_fwd), n, n), dtype=torch.float32, device="cuda")
+    full = (n,) * dim
+    half = (n,) * (dim - 1) + (nh,)
+    axes = tuple(range(-dim, 0))
+    Q = torch.zeros((max(n_inv, n_fwd),) + half, dtype=torch.complex64, device="cuda")
+    R = torch.zeros((max(n_inv, n_fwd),) + full, dtype=torch.float32, device="cuda")
 
     def stage() -> None:
-        torch.fft.irfftn(Q[:n_inv], s=(n, n), dim=(-2, -1), out=R[:n_inv])
-        torch.fft.rfftn(R[:n_fwd], dim=(-2, -1), out=Q[:n_fwd])
+        torch.fft.irfftn(Q[:n_inv], s=full, dim=axes, out=R[:n_inv])
+        torch.fft.rfftn(R[:n_fwd], dim=axes, out=Q[:n_fwd])
 
     for _ in range(3):
         stage()
@@ -97,15 +121,16 @@ def rocfft_stage(torch, n: int, n_inv: int = 2, n_fwd: int = 2, repeats: int = 2
     return float(np.median(ts))
 
 
-def cpu_baseline(n: int, state: np.ndarray, dt: float, budget_s: float = 30.0) -> dict:
+def cpu_baseline(n: int, state: np.ndarray, dt: float, budget_s: float = 30.0, dim: int = 2,
+                 nu: float = 1e-5) -> dict:
     """The CPU oracle (scipy.fft, fp32, workers = this job's CPU share) on a bounded sample of
-    the same workload: whole IF-RK4 steps of the same 8192^2 state and dt."""
+    the same workload: whole IF-RK4 steps of the same n^dim state and dt."""
     from oracle import numpy_oracle as O
     from pyspectral_amd.config import Config
 
     cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
                 or (os.cpu_count() or 1))
-    s = O.OracleSolver(Config(n=n, dim=2, nu=1e-5, dtype="float32"), workers=cores)
+    s = O.OracleSolver(Config(n=n, dim=dim, nu=nu, dtype="float32"), workers=cores)
     s.set_state(state)
     times = []
     t_start = time.perf_counter()
@@ -116,8 +141,8 @@ def cpu_baseline(n: int, state: np.ndarray, dt: float, budget_s: float = 30.0) -
         if time.perf_counter() - t_start + times[-1] > budget_s or len(times) >= 3:
             break
     t = min(times[1:]) if len(times) > 1 else times[0]
-    return {"value": n * n / t / 1e6, "unit": "Mpts/s", "cores": cores, "kind": "port",
-            "sample": f"{len(times)} IF-RK4 step(s) of the same {n}^2 fp32 state and dt "
+    return {"value": n ** dim / t / 1e6, "unit": "Mpts/s", "cores": cores, "kind": "port",
+            "sample": f"{len(times)} IF-RK4 step(s) of the same {n}^{dim} fp32 state and dt "
                       f"(oracle/numpy_oracle.py, scipy.fft workers={cores}); best of the "
                       f"steps after the first: {t:.2f} s/step",
             "s_per_step": t, "cpu": platform.processor() or platform.machine()}
@@ -238,6 +263,94 @@ def ensemble_cpu_baseline(cfgs, states, outs) -> dict:
                       "counts: %s s per member step" % (cores, per)}
 
 
+def tgv3d_bench(a: argparse.Namespace) -> int:
+    """The 3D Taylor-Green vortex at Re = 1600 on n^3 (default 384^3), fp32, Basdevant form
+    (8 FFTs per stage), CFL 0.5 with dt every 10 steps on the device."""
+    import torch
+
+    from oracle.numpy_oracle import traffic_model
+    from pyspectral_amd import _lib, initial
+    from pyspectral_amd.backends import get
+    from pyspectral_amd.config import Config
+
+    n = a.n3
+    nu = 1.0 / 1600.0
+    cfg = Config(n=n, dim=3, nu=nu, dtype="float32")
+    w0 = initial.to_spectral(initial.taylor_green_3d(n), 3).astype(np.complex64)
+    s = get("hip/fused").make(cfg)
+    s.set_state(w0)
+    lay = s.buf.layout
+    run_kw = dict(cfl=0.5, dt_max=0.05, cfl_every=10, chunk=10, record_start=False)
+    s.run_device(math.inf, kmax=a.warmup, use_graph=False, **run_kw)
+    torch.cuda.synchronize()
+    lib, ctx, st = s.lib, s.ctx, _lib.stream_ptr()
+    _lib.check(lib.psx_timing_begin(ctx))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run = s.run_device(math.inf, kmax=a.steps, use_graph=False, **run_kw)
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / a.steps
+    nk = len(_lib.KINDS)
+    ms = (ctypes.c_double * nk)()
+    cnt = (ctypes.c_int * nk)()
+    _lib.check(lib.psx_timing_end(ctx, st, ms, cnt))
+    per_kernel = {_lib.KINDS[i]: {"ms_total": ms[i], "launches": cnt[i],
+                                  "us_per_launch": 1e3 * ms[i] / cnt[i]}
+                  for i in range(nk) if cnt[i]}
+    s.set_state(w0)
+    s.t = 0.0
+    s.run_device(math.inf, kmax=20, use_graph=True, **run_kw)
+    torch.cuda.synchronize()
+    tg0 = time.perf_counter()
+    s.run_device(math.inf, kmax=a.steps, use_graph=True, **run_kw)
+    torch.cuda.synchronize()
+    t_graph = (time.perf_counter() - tg0) / a.steps
+    finite = bool(torch.isfinite(torch.view_as_real(s.state_native())).all().item())
+    kb = kernel_bytes_3d(n, lay.jm, lay.ldw, False, 8)
+    dom = max((k for k in per_kernel if k in kb), key=lambda k: per_kernel[k]["ms_total"])
+    dom_s = per_kernel[dom]["ms_total"] / 1e3 / per_kernel[dom]["launches"]
+    achieved = kb[dom] / dom_s / 1e9
+    step_bytes = sum(kb[k] * per_kernel[k]["launches"] for k in per_kernel if k in kb) / a.steps
+    tm = traffic_model(n, 3, "basdevant", "float32")
+    value = n ** 3 / t_step / 1e6
+    out = {
+        "metric": "Mpts/s per RK4 step", "value": value, "unit": "Mpts/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": t_step * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": value / REF_MPTS_3D,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"3D Taylor-Green vortex Re=1600 {n}^3 fp32, IF-RK4 Basdevant "
+                               "(8 FFTs per stage), 2/3 dealias, Leray projection, CFL 0.5 "
+                               "(dt every 10 steps, device-resident loop)",
+                   "n": n, "nu": nu, "cfl": 0.5, "variant": "hip/fused"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_launch": kb[dom],
+                     "avg_launch_us": dom_s * 1e6},
+        "per_kernel": per_kernel,
+        "step_hbm_bytes_algorithmic": step_bytes, "step_hbm_GBs": step_bytes / t_step / 1e9,
+        "reference_model_bytes_per_step": tm["total_bytes"],
+        "reference_model_effective_GBs": tm["total_bytes"] / t_step / 1e9,
+        "graph_ms_per_step": t_graph * 1e3, "dt_first": run["dt"][0], "finite": finite,
+        "vs_baseline_note": "reference 310 Mpts/s: cupy-cuda-c/fused-inplace-lean 384^3 fp32, "
+                            "RTX 3070 Laptop",
+    }
+    dt0 = run["dt"][0]
+    s.close()
+    del s
+    torch.cuda.empty_cache()
+    if not a.no_roofline:
+        bcopy = copy_bandwidth(torch)
+        t_stage = rocfft_stage(torch, n, n_inv=3, n_fwd=5, repeats=10, dim=3)
+        t_roof = 4 * t_stage + tm["pointwise_bytes"] / (bcopy * 1e9)
+        out.update({"pct_fft_roofline": 100 * t_roof / t_step, "roofline_step_ms": t_roof * 1e3,
+                    "rocfft_stage_ms": t_stage * 1e3, "copy_bandwidth_GBs": bcopy})
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n, w0, dt0, dim=3, nu=nu)
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+    return 0 if finite else 1
+
+
 def main(argv: list[str] | None = None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -246,8 +359,10 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--n", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-roofline", action="store_true", help="skip rocFFT/copy ceilings")
-    ap.add_argument("--workload", choices=("single", "ensemble"), default="single",
-                    help="single: one 8192^2 flow (default); ensemble: the forced ensemble")
+    ap.add_argument("--n3", type=int, default=384, help="grid of the tgv3d workload")
+    ap.add_argument("--workload", choices=("single", "ensemble", "tgv3d"), default="single",
+                    help="single: one 8192^2 flow (default); ensemble: the forced ensemble; "
+                         "tgv3d: the 3D Taylor-Green vortex on n3^3")
     a = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -258,6 +373,8 @@ def main(argv: list[str] | None = None) -> int:
 
     if a.workload == "ensemble":
         return ensemble_bench(a)
+    if a.workload == "tgv3d":
+        return tgv3d_bench(a)
 
     import torch
 

@@ -96,7 +96,8 @@ int psx_step_graph(psx_ctx* c, void* stream);
 #define PSX_KIND_MID_FWD 4   /* 3D middle-axis forward pass */
 #define PSX_KIND_MID_INV 5   /* 3D middle-axis inverse pass */
 #define PSX_KIND_CTRL 6      /* device time-loop controller and its record reduction */
-#define PSX_NKINDS 7
+#define PSX_KIND_SGS 7       /* LES subgrid-stress evaluation at the start of a step */
+#define PSX_NKINDS 8
 
 /* Per-launch GPU timing of eager steps with HIP events on the caller's stream: between
  * psx_timing_begin and psx_timing_end every psx_step records an event before each launch;
@@ -210,6 +211,34 @@ int psx_ens_run_end(psx_ens* e, void* stream);
  * psx_set_source: a 3D context adds src (state layout, or null) to the nonlinear term of every
  * stage; the LES adds the Leray-projected divergence of its subgrid stress this way. */
 int psx_set_source(psx_ctx* c, void* src);
+
+/* psx_set_sgs: a 3D fp32 context (L = n = 64, 128 or 256) evaluates a subgrid-stress model
+ * from its state at the start of every step -- eager, graph-replayed and controller-driven
+ * steps alike -- and adds the result through the source above: velocity gradients by
+ * spectral differentiation and inverse 3D FFTs, the stress (Smagorinsky, or the network's six
+ * convolutions on MFMA), its forward FFT and the Leray-projected divergence -i k_j tau_ij.
+ * Every buffer belongs to the caller and must stay valid while attached; d = NULL detaches
+ * (and clears the source).  psx_sgs_update enqueues the same evaluation once, outside a step.
+ * Sizes are complex64 counts over the L^3 grid, and [L^3][32] point-major channel rows. */
+#define PSX_SGS_SMAGORINSKY 1
+#define PSX_SGS_NET 2
+typedef struct psx_sgs_desc {
+  int model;          /* PSX_SGS_SMAGORINSKY or PSX_SGS_NET */
+  void* src;          /* the source, state layout (3 n^2 rows of the state pitch) */
+  void* full;         /* scratch: 3 L^3 */
+  void* grad;         /* scratch: 6 L^3 */
+  void* tau;          /* scratch: 3 L^3 */
+  void* x;            /* network input, bf16 [L^3][32] */
+  const void* w[6];   /* network weights, bf16 [27][32 out][32 in] per layer */
+  const float* b[6];  /* network biases, fp32 [32] per layer */
+  void* act[5];       /* hidden activations, bf16 [L^3][32] */
+  float* out;         /* network output, fp32 [L^3][32] (channels 0..5 used) */
+  double g_scale;     /* input scale: 1 / velocity-gradient rms */
+  double tau_scale;   /* output scale: stress rms */
+  double smag;        /* Smagorinsky (Cs Delta)^2 */
+} psx_sgs_desc;
+int psx_set_sgs(psx_ctx* c, const psx_sgs_desc* d);
+int psx_sgs_update(psx_ctx* c, void* stream);
 
 /* In-place unnormalised 3D complex FFT (complex64) of nbatch contiguous L^3 cubes
  * (L = 64, 128, 256; z fastest); inverse = 1 for exp(+i k.x). */

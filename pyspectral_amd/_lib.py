@@ -58,6 +58,15 @@ class AdamArgs(ctypes.Structure):
                 ("c2", c_double), ("is_w", c_int)]
 
 
+class SgsDesc(ctypes.Structure):
+    _fields_ = [("model", c_int), ("src", c_void_p), ("full", c_void_p), ("grad", c_void_p),
+                ("tau", c_void_p), ("x", c_void_p), ("w", c_void_p * 6), ("b", c_void_p * 6),
+                ("act", c_void_p * 5), ("out", c_void_p), ("g_scale", c_double),
+                ("tau_scale", c_double), ("smag", c_double)]
+
+
+SGS_MODELS = {"smagorinsky": 1, "net": 2}  # PSX_SGS_*
+
 LES_OPS = {"expand": 0, "gradpack": 1, "products": 2, "truncate": 3, "pairup": 4,
            "features": 5, "smagorinsky": 6, "tau_from_net": 7, "target": 8, "source": 9,
            "sumsq": 10}  # PSX_LES_*
@@ -102,6 +111,8 @@ SIGNATURES = {
     "psx_ens_run_poll": (c_int, [c_void_p, POINTER(RunState), c_void_p, c_void_p, c_void_p]),
     "psx_ens_run_end": (c_int, [c_void_p, c_void_p]),
     "psx_set_source": (c_int, [c_void_p, c_void_p]),
+    "psx_set_sgs": (c_int, [c_void_p, POINTER(SgsDesc)]),
+    "psx_sgs_update": (c_int, [c_void_p, c_void_p]),
     "psx_fft3": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
     "psx_les_op": (c_int, [c_int, POINTER(LesArgs), c_void_p]),
     "psx_cnn_conv": (c_int, [POINTER(ConvArgs), c_void_p]),
@@ -113,7 +124,7 @@ SIGNATURES = {
     "psx_device_info": (c_int, [c_char_p, c_int]),
 }
 
-KINDS = ("line", "outer_fwd", "spec", "outer_inv", "mid_fwd", "mid_inv", "ctrl")  # PSX_KIND_*
+KINDS = ("line", "outer_fwd", "spec", "outer_inv", "mid_fwd", "mid_inv", "ctrl", "sgs")  # PSX_KIND_*
 
 _LIB: ctypes.CDLL | None = None
 

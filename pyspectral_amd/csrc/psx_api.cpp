@@ -12,6 +12,7 @@
 
 #include "psx_geo.h"
 #include "psx_host.h"
+#include "psx_les.h"
 #include "psx_kernels.h"
 
 using namespace psx;
@@ -80,7 +81,7 @@ bool plan_rows(int L, int nrows, bool fp64, bool heavy, int nmax_vals, Launch* l
   int E = pick_E(L, fp64, heavy);
   if (!plan_fft(L, E, p)) return false;
   int t = 1;
-  while (L * t / E < 128 && t * 2 <= nrows) t *= 2;
+  while (L * t / E < 128 && nrows % (t * 2) == 0) t *= 2;  // t divides nrows
   int nt = L * t / E;
   if (nt > 512) {
     if (E == 16 && !fp64 && plan_fft(L, 32, p)) {
@@ -148,6 +149,9 @@ struct psx_ctx {
   DrvState* hslots = nullptr;
   hipEvent_t slot_ev[kSlots] = {};
   long long nchunk = 0;
+  // LES subgrid-stress model evaluated at the start of every step (psx_set_sgs)
+  psx_sgs_desc sgs{};
+  bool has_sgs = false;
   hipGraph_t rgraph = nullptr;
   hipGraphExec_t rgexec = nullptr;
   bool ct = false;  // compile-time specialised fp32 kernels for this size (2D or 3D)
@@ -215,6 +219,7 @@ int plan_flow(const psx_flow_params& p, psx_ctx* c, Flow<T>* f) {
     conv(g.line, &f->prow);
     conv(g.spec, &f->pcol);
     if (g.n1 > 1) conv(g.outer, &f->pa);
+    if (n % g.lineT) return fail("psx: line tile of %d rows does not divide n=%d", g.lineT, n);
     c->line.E = g.lineE;
     c->line.threads = g.lineNT;
     c->line.blocks = n / g.lineT;
@@ -253,6 +258,7 @@ int plan_flow(const psx_flow_params& p, psx_ctx* c, Flow<T>* f) {
     conv(g.yp, &f->py);
     conv(g.xp, &f->pcol);
     const int njt = g.ldw / g.T;
+    if (nrows % g.lineT) return fail("psx: line tile of %d rows does not divide n^2", g.lineT);
     c->line.E = g.lineE;
     c->line.threads = g.lineNT;
     c->line.blocks = nrows / g.lineT;
@@ -362,6 +368,13 @@ int enqueue_prepare(psx_ctx* c, hipStream_t st) {
 template <typename T>
 int enqueue_step(psx_ctx* c, hipStream_t st, int drv = 0) {
   Flow<T>& f = flow<T>(c);
+  if constexpr (sizeof(T) == 4) {
+    if (c->has_sgs) {
+      PSX_MARK(PSX_KIND_SGS);
+      const int r = enqueue_sgs(c->sgs, f.u0, f.sp, f.n, st);
+      if (r) return r;
+    }
+  }
   for (int s = 0; s < 4; ++s) {
     const int lflags = (s == 0 && drv == 2) ? kLineWantMax : 0;
     bool ct = false;
@@ -708,6 +721,36 @@ int psx_set_source(psx_ctx* c, void* src) {
   c->graph = c->rgraph = nullptr;
   c->run_chunk = 0;
   return 0;
+}
+
+int psx_set_sgs(psx_ctx* c, const psx_sgs_desc* d) {
+  if (!c) return fail("psx_set_sgs: null context");
+  if (!d) {
+    c->has_sgs = false;
+    return psx_set_source(c, nullptr);
+  }
+  if (c->p.dim != 3 || c->fp64) return fail("psx_set_sgs: 3D fp32 flows only");
+  if (!c2c_supported(c->p.n)) return fail("psx_set_sgs: n=%d unsupported (64, 128, 256)", c->p.n);
+  if (d->model != PSX_SGS_SMAGORINSKY && d->model != PSX_SGS_NET)
+    return fail("psx_set_sgs: unknown model %d", d->model);
+  if (!d->src || !d->full || !d->grad || !d->tau) return fail("psx_set_sgs: null buffer");
+  if (d->model == PSX_SGS_NET) {
+    if (!d->x || !d->out) return fail("psx_set_sgs: null network buffer");
+    for (int i = 0; i < 6; ++i)
+      if (!d->w[i] || !d->b[i] || (i < 5 && !d->act[i]))
+        return fail("psx_set_sgs: null network tensor (layer %d)", i);
+  }
+  const int r = psx_set_source(c, d->src);  // also drops the captured graphs
+  if (r) return r;
+  c->sgs = *d;
+  c->has_sgs = true;
+  return 0;
+}
+
+int psx_sgs_update(psx_ctx* c, void* stream) {
+  if (!c) return fail("psx_sgs_update: null context");
+  if (!c->has_sgs) return fail("psx_sgs_update: no subgrid-stress model attached");
+  return enqueue_sgs(c->sgs, c->f32.u0, c->f32.sp, c->f32.n, static_cast<hipStream_t>(stream));
 }
 
 // ------------------------------------------------------------------ device controller

@@ -101,11 +101,13 @@ constexpr Geo2D geo2d(int n, bool fp64) {
   while (T > 1 && T / 2 >= g.jm) T /= 2;
   g.colT = T;
   g.ldw = (g.jm + T - 1) / T * T;
-  // line kernel: whole rows (long fp32 rows: 32 elements per thread, 3 passes at 8192)
+  // line kernel: whole rows (long fp32 rows: 32 elements per thread, 3 passes at 8192); the
+  // rows per work-group divide n, so every launch covers the rows exactly (n = 3 * 2^a
+  // included)
   {
     int E = (!fp64 && n % 3 != 0 && n >= 4096) ? 32 : ct_pick_E(n, fp64);
     int t = 1;
-    while (n * t / E < 128 && t * 2 <= n) t *= 2;
+    while (n * t / E < 128 && n % (t * 2) == 0) t *= 2;
     int nt = n * t / E;
     if (nt > 512 && !fp64 && n % 3 != 0) {
       E = 32;
@@ -235,7 +237,7 @@ constexpr Geo3D geo3d(int n) {
   {
     const int E = r3 ? 24 : 16;
     int t = 1;
-    while (n * t / E < 128 && t * 2 <= n * n) t *= 2;
+    while (n * t / E < 128 && (n * n) % (t * 2) == 0) t *= 2;  // t divides the n^2 rows
     const int nt = n * t / E;
     g.line = ct_plan(n, E);
     if (g.line.np == 0 || nt > 512 || (n * t) % E) return g;

@@ -72,121 +72,134 @@ __global__ void __launch_bounds__(256) k_conv(ConvArgs a) {
     }
 }
 
+// ------------------------------------------------- forward / input grad, n = 64, LDS-staged
+// Work-group: a brick of 4 (x) x 4 (y) x 32 (z) output points of one cube, all 32 output
+// channels.  The brick and its one-point periodic halo (6 x 6 x 34 points of 32 bf16
+// channels, 64 B each) are staged in LDS once, so each input point is read from memory
+// about 2.4 times instead of 27; every tap's A operand is then an LDS read.  The four 16-byte
+// channel groups of halo point q are rotated by q >> 1: the 8 consecutive points of one MFMA
+// row group then fall in distinct banks for any tap offset.  Wave w owns the brick lines
+// (x = w, y = 0..3): 8 row tiles of 16 z points.  The result goes through LDS once more
+// (fp32, padded rows) so that the epilogue (bias, ReLU, mask) reads and writes whole 16-byte
+// channel groups of consecutive points.
+constexpr int kCvBx = 4, kCvBy = 4, kCvBz = 32;
+constexpr int kCvHx = kCvBx + 2, kCvHy = kCvBy + 2, kCvHz = kCvBz + 2;
+constexpr int kCvHalo = kCvHx * kCvHy * kCvHz;       // 1224 points
+constexpr int kCvPts = kCvBx * kCvBy * kCvBz;        // 512 points
+constexpr int kCvLds = kCvHalo * 64;                 // 78336 bytes: two work-groups per CU
+constexpr int kCvOutPitch = kCnnC + 4;               // fp32 staging row, 144 bytes
+static_assert(kCvPts * kCvOutPitch * 4 <= kCvLds, "output staging fits the halo buffer");
+
+__device__ __forceinline__ int cv_slot(int q, int g) {
+  return q * 64 + (((g + (q >> 1)) & 3) << 4);
+}
+
+__global__ void __launch_bounds__(256) k_conv64(ConvArgs a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int n = 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r = lane & 15, kg = lane >> 4;
+  // bricks are dispatched round-robin over the 8 XCDs: give each XCD a contiguous range of
+  // bricks, so that neighbouring bricks (which share halo points) share an L2
+  const int per = (int)gridDim.x >> 3;
+  const int brick = ((int)blockIdx.x & 7) * per + ((int)blockIdx.x >> 3);
+  const int cube = brick >> 9;
+  const int x0 = ((brick >> 5) & 15) * kCvBx, y0 = ((brick >> 1) & 15) * kCvBy,
+            z0 = (brick & 1) * kCvBz;
+  const unsigned char* X =
+      reinterpret_cast<const unsigned char*>(a.x) + (size_t)cube * (n * n * n * 64);
+  for (int c = tid; c < kCvHalo * 4; c += 256) {
+    const int q = c >> 2, g = c & 3;
+    const int hz = q % kCvHz, hy = (q / kCvHz) % kCvHy, hx = q / (kCvHz * kCvHy);
+    const int gx = (x0 - 1 + hx) & (n - 1), gy = (y0 - 1 + hy) & (n - 1),
+              gz = (z0 - 1 + hz) & (n - 1);
+    const uint4 v = *reinterpret_cast<const uint4*>(X + ((((gx << 6) | gy) << 6 | gz) << 6) +
+                                                    (g << 4));
+    *reinterpret_cast<uint4*>(smem_raw + cv_slot(q, g)) = v;
+  }
+  __syncthreads();
+
+  f32x4 acc[8][2];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) acc[t][0] = acc[t][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bf16x8* W = reinterpret_cast<const bf16x8*>(a.w);
+  // halo point of row r of tile t = 2 ly + zt at tap (dx, dy, dz):
+  // ((wave + 1 + dx) * 6 + ly + 1 + dy) * 34 + 16 zt + r + 1 + dz
+  const int qlane = ((wave + 1) * kCvHy + 1) * kCvHz + r + 1;
+#pragma unroll
+  for (int tap = 0; tap < kCnnTaps; ++tap) {
+    const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dz = tap % 3 - 1;
+    const bf16x8 b0 = W[(tap * kCnnC + r) * 4 + kg];
+    const bf16x8 b1 = W[(tap * kCnnC + 16 + r) * 4 + kg];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int q = qlane + (dx * kCvHy + (t >> 1) + dy) * kCvHz + 16 * (t & 1) + dz;
+      const bf16x8 av = *reinterpret_cast<const bf16x8*>(smem_raw + cv_slot(q, kg));
+      acc[t][0] = mfma(av, b0, acc[t][0]);
+      acc[t][1] = mfma(av, b1, acc[t][1]);
+    }
+  }
+  __syncthreads();  // every wave is done with the halo
+
+  // accumulators -> LDS rows of local point lp = (lx * 4 + ly) * 32 + z
+  float* so = reinterpret_cast<float*>(smem_raw);
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int lp0 = (wave * kCvBy + (t >> 1)) * kCvBz + 16 * (t & 1) + 4 * kg;
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) so[(lp0 + j) * kCvOutPitch + 16 * u + r] = acc[t][u][j];
+  }
+  __syncthreads();
+  // epilogue: (local point, 8-channel group) items, consecutive items contiguous in memory
+  for (int it = tid; it < kCvPts * 4; it += 256) {
+    const int lp = it >> 2, g = it & 3;
+    const int z = lp & (kCvBz - 1), ly = (lp >> 5) & 3, lx = lp >> 7;
+    const size_t p = (((size_t)cube * n + x0 + lx) * n + y0 + ly) * n + z0 + z;
+    const float4 lo = *reinterpret_cast<const float4*>(so + lp * kCvOutPitch + 8 * g);
+    const float4 hi = *reinterpret_cast<const float4*>(so + lp * kCvOutPitch + 8 * g + 4);
+    float v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+    if (a.bias) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] += a.bias[8 * g + i];
+    }
+    if (a.relu) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) v[i] = fmaxf(v[i], 0.f);
+    }
+    if (a.mask) {
+      const bf16x8 m = *reinterpret_cast<const bf16x8*>(a.mask + p * kCnnC + 8 * g);
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (!((float)m[i] > 0.f)) v[i] = 0.f;
+    }
+    if (a.y) {
+      bf16x8 o;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) o[i] = (__bf16)v[i];
+      *reinterpret_cast<bf16x8*>(a.y + p * kCnnC + 8 * g) = o;
+    }
+    if (a.yf) {
+      *reinterpret_cast<float4*>(a.yf + p * kCnnC + 8 * g) = float4{v[0], v[1], v[2], v[3]};
+      *reinterpret_cast<float4*>(a.yf + p * kCnnC + 8 * g + 4) =
+          float4{v[4], v[5], v[6], v[7]};
+    }
+  }
+}
+
 hipError_t launch_conv(const ConvArgs& a, hipStream_t st) {
   if (a.n < 4 || (a.n & (a.n - 1)) || a.nb < 1 || !a.x || !a.w || (!a.y && !a.yf))
     return hipErrorInvalidValue;
   const long long P = (long long)a.nb * a.n * a.n * a.n;
+  if (a.n == 64) {
+    // 512 bricks per cube (a multiple of the 8 XCDs)
+    hipLaunchKernelGGL(k_conv64, dim3((unsigned)(P / kCvPts)), dim3(256), kCvLds, st, a);
+    return hipGetLastError();
+  }
   const unsigned blocks = (unsigned)((P + 255) / 256);
   hipLaunchKernelGGL(k_conv, dim3(blocks), dim3(256), 0, st, a);
   return hipGetLastError();
-}
-
-// ----------------------------------------------------------------------- weight gradient
-// Work-group g accumulates rows (cube, x, y) g, g + G, ... (each a z line of n points): the
-// output gradient line and the 9 neighbouring input lines (dx, dy) are staged channel-major
-// in LDS with a one-point periodic halo, so that an MFMA operand (8 consecutive points of one
-// channel) is one contiguous read.  Wave w owns taps w, w + 4, ...; D[out 16][in 16] tiles.
-// Partial sums go to part[g][27 * 32 * 32 + 32] (the last 32: bias gradient).
-constexpr int kWgLine = 64;                  // points per line (n = 64)
-constexpr int kWgPitch = kWgLine + 8;        // halo of 1 each side, padded
-constexpr int kWgTapsPerWave = 7;
-
-__global__ void __launch_bounds__(256) k_conv_wgrad(const __bf16* __restrict__ x,
-                                                    const __bf16* __restrict__ dz, int nb,
-                                                    float* __restrict__ part) {
-  constexpr int n = kWgLine, ln = 6;
-  __shared__ __bf16 xs[9][kCnnC][kWgPitch];  // [dx,dy][channel][1 + z]
-  __shared__ __bf16 gs[kCnnC][kWgPitch];     // [out channel][z]
-  __shared__ float bsum[4][kCnnC];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int r = lane & 15, kg = lane >> 4;
-  const long long nlines = (long long)nb * n * n;
-  f32x4 acc[kWgTapsPerWave][2][2];
-#pragma unroll
-  for (int i = 0; i < kWgTapsPerWave; ++i)
-#pragma unroll
-    for (int u = 0; u < 2; ++u) acc[i][u][0] = acc[i][u][1] = f32x4{0.f, 0.f, 0.f, 0.f};
-  float bacc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  const int sp = tid >> 2, sc = (tid & 3) * 8;  // staging: point sp, channels sc..sc+7
-  for (long long line = blockIdx.x; line < nlines; line += gridDim.x) {
-    const long long base = line * n;  // point index of z = 0
-    __syncthreads();
-    {
-      const bf16x8 gv = *reinterpret_cast<const bf16x8*>(dz + (base + sp) * kCnnC + sc);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        gs[sc + j][sp] = gv[j];
-        bacc[j] += (float)gv[j];
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < 9; ++s) {
-      const int dx = s / 3 - 1, dy = s % 3 - 1;
-      const long long q = nbr(base + sp, ln, dx, dy, 0);
-      const bf16x8 xv = *reinterpret_cast<const bf16x8*>(x + q * kCnnC + sc);
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        xs[s][sc + j][1 + sp] = xv[j];
-        if (sp == 0) xs[s][sc + j][1 + n] = xv[j];       // halo z = n  -> z = 0
-        if (sp == n - 1) xs[s][sc + j][0] = xv[j];       // halo z = -1 -> z = n - 1
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < n / 32; ++kk) {
-      const int z0 = 32 * kk + 8 * kg;  // this lane's 8 points: z0 .. z0 + 7
-      bf16x8 ga[2];
-#pragma unroll
-      for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int j = 0; j < 8; ++j) ga[u][j] = gs[16 * u + r][z0 + j];
-#pragma unroll
-      for (int i = 0; i < kWgTapsPerWave; ++i) {
-        const int tap = wave + 4 * i;
-        if (tap < kCnnTaps) {
-          const int s = tap / 3, dzz = tap % 3 - 1;
-#pragma unroll
-          for (int v = 0; v < 2; ++v) {
-            bf16x8 xb;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) xb[j] = xs[s][16 * v + r][1 + z0 + j + dzz];
-            acc[i][0][v] = mfma(ga[0], xb, acc[i][0][v]);
-            acc[i][1][v] = mfma(ga[1], xb, acc[i][1][v]);
-          }
-        }
-      }
-    }
-  }
-  // partial sums of this work-group
-  float* out = part + (long long)blockIdx.x * (kCnnTaps * kCnnC * kCnnC + kCnnC);
-#pragma unroll
-  for (int i = 0; i < kWgTapsPerWave; ++i) {
-    const int tap = wave + 4 * i;
-    if (tap < kCnnTaps) {
-#pragma unroll
-      for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int v = 0; v < 2; ++v)
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const int o = 16 * u + 4 * kg + j, ic = 16 * v + r;
-            out[(tap * kCnnC + o) * kCnnC + ic] = acc[i][u][v][j];
-          }
-    }
-  }
-  // bias gradient: threads with the same channel group (tid & 3) hold channels sc..sc+7
-  __syncthreads();
-  float* red = reinterpret_cast<float*>(&xs[0][0][0]);  // reuse: 256 x 8 floats
-#pragma unroll
-  for (int j = 0; j < 8; ++j) red[tid * 8 + j] = bacc[j];
-  __syncthreads();
-  if (tid < kCnnC) {
-    const int grp = tid / 8, j = tid % 8;
-    float sacc = 0.f;
-    for (int t = grp; t < 256; t += 4) sacc += red[t * 8 + j];
-    out[kCnnTaps * kCnnC * kCnnC + tid] = sacc;
-  }
-  (void)bsum;
 }
 
 // Sum of the per-work-group partials (fixed order), added into dw / db.
@@ -201,18 +214,151 @@ __global__ void k_wgrad_reduce(const float* __restrict__ part, int ngroups, floa
   else db[i - NW] += s;
 }
 
+// ------------------------------------------------------ weight gradient, n = 64, MFMA-only
+// dW[tap][o][i] = sum_p dz[p][o] x[p + tap][i] is a contraction over points, so both MFMA
+// operands need points along k, while the tensors are point-major (channels contiguous).
+// The transposition is itself an MFMA: a 16-point x 32-channel row tile (the natural
+// point-major operand) times the 32 x 16 identity slice I_u (I_u[k][n] = [k == 16 u + n])
+// leaves lane (r, kg) holding the tile's points 4 kg + j, j < 4, of channel 16 u + r -- one
+// product by 1.0 each, so exact.  Two tiles (a 32-point line segment) give each lane 8
+// points of one channel: an operand with k = points, in the same point order for dz (A,
+// rows = output channels) and for x (B, columns = input channels), which is all the
+// contraction needs.  The shifted x tiles come from the LDS halo brick of the forward kernel.
+// Work-groups persist over bricks (4 x 4 x 32 points); wave w accumulates taps w, w + 4, ...
+// (7 or 6 taps, 32 x 32 each) over all points of its bricks, and writes per-work-group
+// partial sums (fixed-order reduction afterwards, as above).
+constexpr int kWg64Grid = 512;
+
+__global__ void __launch_bounds__(256) k_wgrad64(const __bf16* __restrict__ x,
+                                                 const __bf16* __restrict__ dz, int nbricks,
+                                                 float* __restrict__ part) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  constexpr int n = 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r = lane & 15, kg = lane >> 4;
+  bf16x8 id[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) id[u][j] = (__bf16)((8 * kg + j == 16 * u + r) ? 1.f : 0.f);
+  const f32x4 zero4 = f32x4{0.f, 0.f, 0.f, 0.f};
+  f32x4 acc[7][2][2];
+#pragma unroll
+  for (int i = 0; i < 7; ++i)
+#pragma unroll
+    for (int u = 0; u < 2; ++u) acc[i][u][0] = acc[i][u][1] = zero4;
+  float bsum[2] = {0.f, 0.f};
+  const bf16x8* DZ = reinterpret_cast<const bf16x8*>(dz);
+
+  for (int brick = blockIdx.x; brick < nbricks; brick += gridDim.x) {
+    const int cube = brick >> 9;
+    const int x0 = ((brick >> 5) & 15) * kCvBx, y0 = ((brick >> 1) & 15) * kCvBy,
+              z0 = (brick & 1) * kCvBz;
+    const unsigned char* X =
+        reinterpret_cast<const unsigned char*>(x) + (size_t)cube * (n * n * n * 64);
+    __syncthreads();  // the previous brick's halo is no longer read
+    for (int c = tid; c < kCvHalo * 4; c += 256) {
+      const int q = c >> 2, g = c & 3;
+      const int hz = q % kCvHz, hy = (q / kCvHz) % kCvHy, hx = q / (kCvHz * kCvHy);
+      const int gx = (x0 - 1 + hx) & (n - 1), gy = (y0 - 1 + hy) & (n - 1),
+                gz = (z0 - 1 + hz) & (n - 1);
+      const uint4 v = *reinterpret_cast<const uint4*>(
+          X + ((((gx << 6) | gy) << 6 | gz) << 6) + (g << 4));
+      *reinterpret_cast<uint4*>(smem_raw + cv_slot(q, g)) = v;
+    }
+    __syncthreads();
+    for (int line = 0; line < kCvBx * kCvBy; ++line) {
+      const int lx = line >> 2, ly = line & 3;
+      // dz of the 32-point segment, transposed: A[u] = channel 16 u + r, 8 points
+      bf16x8 A[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const size_t p =
+            (((size_t)cube * n + x0 + lx) * n + y0 + ly) * n + z0 + 16 * h + r;
+        const bf16x8 rows = DZ[p * 4 + kg];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const f32x4 t = mfma(rows, id[u], zero4);
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            A[u][4 * h + j] = (__bf16)t[j];
+            bsum[u] += t[j];
+          }
+        }
+      }
+      const int qline = ((lx + 1) * kCvHy + ly + 1) * kCvHz + r + 1;
+#pragma unroll
+      for (int i = 0; i < 7; ++i) {
+        const int tap = wave + 4 * i;
+        if (tap < kCnnTaps) {
+          const int dx = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dzz = tap % 3 - 1;
+          bf16x8 B[2];
+#pragma unroll
+          for (int h = 0; h < 2; ++h) {
+            const int q = qline + (dx * kCvHy + dy) * kCvHz + 16 * h + dzz;
+            const bf16x8 rows = *reinterpret_cast<const bf16x8*>(smem_raw + cv_slot(q, kg));
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+              const f32x4 t = mfma(rows, id[v], zero4);
+#pragma unroll
+              for (int j = 0; j < 4; ++j) B[v][4 * h + j] = (__bf16)t[j];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int v = 0; v < 2; ++v) acc[i][u][v] = mfma(A[u], B[v], acc[i][u][v]);
+        }
+      }
+    }
+  }
+
+  // partial sums of this work-group: acc[i][u][v][j] = dW[tap][16 u + 4 kg + j][16 v + r]
+  float* out = part + (long long)blockIdx.x * (kCnnTaps * kCnnC * kCnnC + kCnnC);
+#pragma unroll
+  for (int i = 0; i < 7; ++i) {
+    const int tap = wave + 4 * i;
+    if (tap < kCnnTaps) {
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            out[(tap * kCnnC + 16 * u + 4 * kg + j) * kCnnC + 16 * v + r] = acc[i][u][v][j];
+    }
+  }
+  // bias gradient: wave 0's lanes (r, kg) hold channel 16 u + r summed over their points
+  __syncthreads();
+  float* red = reinterpret_cast<float*>(smem_raw);
+  if (wave == 0) {
+    red[lane * 2] = bsum[0];
+    red[lane * 2 + 1] = bsum[1];
+  }
+  __syncthreads();
+  if (tid < kCnnC) {
+    const int u = tid >> 4, rr = tid & 15;
+    float s = 0.f;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) s += red[(16 * g + rr) * 2 + u];
+    out[kCnnTaps * kCnnC * kCnnC + tid] = s;
+  }
+}
+
 constexpr int kWgGroups = 512;
 
 hipError_t launch_conv_wgrad(const __bf16* x, const __bf16* dz, float* dw, float* db, int n,
                              int nb, hipStream_t st) {
-  if (n != kWgLine || nb < 1) return hipErrorInvalidValue;
+  if (n != 64 || nb < 1) return hipErrorInvalidValue;
+  static_assert(kWg64Grid == kWgGroups, "one partial buffer serves both kernels");
   static float* part = nullptr;  // per-process partial buffer (512 x 27.7k floats)
   if (!part) {
     hipError_t e = hipMalloc(&part, (size_t)kWgGroups * (kCnnTaps * kCnnC * kCnnC + kCnnC) *
                                         sizeof(float));
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_conv_wgrad, dim3(kWgGroups), dim3(256), 0, st, x, dz, nb, part);
+  const int nbricks = nb * (n * n * n / kCvPts);
+  hipLaunchKernelGGL(k_wgrad64, dim3(kWg64Grid), dim3(256), kCvLds, st, x, dz, nbricks, part);
   const int nw = kCnnTaps * kCnnC * kCnnC + kCnnC;
   hipLaunchKernelGGL(k_wgrad_reduce, dim3((nw + 255) / 256), dim3(256), 0, st, part,
                      kWgGroups, dw, db);

@@ -38,6 +38,59 @@ const cx<float>* c2c_twiddles(int L) {
 
 }  // namespace
 
+namespace psx {
+
+int enqueue_sgs(const psx_sgs_desc& d, const cx<float>* u0, int pitch, int L, hipStream_t st) {
+  const cx<float>* tw = c2c_twiddles(L);
+  if (!tw) return fail("sgs: twiddle table allocation failed");
+  auto* full = static_cast<cx<float>*>(d.full);
+  auto* grad = static_cast<cx<float>*>(d.grad);
+  auto* tau = static_cast<cx<float>*>(d.tau);
+  auto op = [&](int which, const void* in, const void* in2, void* out, long long n, int pch,
+                double scale) {
+    LesArgs a;
+    a.cin = static_cast<const cx<float>*>(in);
+    a.in2 = in2;
+    a.cout = static_cast<cx<float>*>(out);
+    a.out = out;
+    a.n = n;
+    a.L = L;
+    a.pitch = pch;
+    a.scale = (float)scale;
+    return launch_les(which, a, st);
+  };
+  // velocity gradients of the state in physical space, six complex pairs
+  PSX_HIP(op(kLesExpand, u0, nullptr, full, 3, pitch, 1.0));
+  PSX_HIP(op(kLesGradPack, full, nullptr, grad, 6, 0, 1.0 / ((double)L * L * L)));
+  PSX_HIP(launch_c2c3(grad, tw, L, 6, true, st));
+  if (d.model == PSX_SGS_SMAGORINSKY) {
+    PSX_HIP(op(kLesSmagorinsky, grad, nullptr, tau, 0, 0, d.smag));
+  } else {
+    PSX_HIP(op(kLesFeatures, grad, nullptr, d.x, 0, 0, d.g_scale));
+    for (int layer = 0; layer < 6; ++layer) {
+      ConvArgs c;
+      c.x = static_cast<const __bf16*>(layer ? d.act[layer - 1] : d.x);
+      c.w = static_cast<const __bf16*>(d.w[layer]);
+      c.bias = d.b[layer];
+      c.n = L;
+      c.nb = 1;
+      if (layer < 5) {
+        c.y = static_cast<__bf16*>(d.act[layer]);
+        c.relu = 1;
+      } else {
+        c.yf = d.out;
+      }
+      PSX_HIP(launch_conv(c, st));
+    }
+    PSX_HIP(op(kLesTauFromNet, nullptr, d.out, tau, 0, 0, d.tau_scale));
+  }
+  PSX_HIP(launch_c2c3(tau, tw, L, 3, false, st));
+  PSX_HIP(op(kLesSource, tau, nullptr, d.src, 0, pitch, 1.0));
+  return 0;
+}
+
+}  // namespace psx
+
 extern "C" {
 
 int psx_fft3(void* data, int L, int nbatch, int inverse, void* stream) {

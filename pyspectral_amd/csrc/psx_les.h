@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/psx.h"
 #include "psx_fft.h"
 
 namespace psx {
@@ -86,5 +87,9 @@ struct AdamArgs {
   int is_w = 0;
 };
 hipError_t launch_adam(const AdamArgs& a, hipStream_t st);
+
+// Subgrid-stress source of the LES model d (psx_set_sgs) from the state u0 (3 components of
+// L^2 rows, row pitch `pitch`) into d.src; 0 or a psx_last_error code.
+int enqueue_sgs(const psx_sgs_desc& d, const cx<float>* u0, int pitch, int L, hipStream_t st);
 
 }  // namespace psx

@@ -10,15 +10,14 @@ No counterpart in the reference.  Pieces, all on libpsx kernels:
   (input and weight gradients) and Adam kernels.
 - ``LESSolver``: the fused 64^3 IF-RK4 step plus the Leray-projected divergence of a subgrid
   stress (none, Smagorinsky, or the network), added to the nonlinear term of every stage.
-  The stress is evaluated from the state at the start of a step (every ``sgs_every`` steps)
-  and held over the step.
+  The C step evaluates the stress from the state at the start of every step and holds it
+  over the step, so the device-resident time loop runs the LES as it runs the DNS.
 """
 
 from __future__ import annotations
 
 import ctypes
 import math
-import time
 from typing import Any
 
 import numpy as np
@@ -236,14 +235,18 @@ class FilterPipeline:
 
 # ------------------------------------------------------------------------------ LES solver
 class LESSolver(HipSolver):
-    """Coarse 3D fp32 flow with a subgrid stress: model in {"none", "smagorinsky", "net"}."""
+    """Coarse 3D fp32 flow with a subgrid stress: model in {"none", "smagorinsky", "net"}.
+
+    The model is attached to the C context (psx_set_sgs): every step -- eager, graph-replayed
+    or run by the device controller -- first evaluates the stress from the state at its start
+    and holds its divergence over the four stages.  ``run_device`` therefore drives an LES
+    exactly as it drives the DNS."""
 
     variant = "les"
-    device_loop = False  # the stress is computed on the host's schedule
 
     def __init__(self, cfg: Config, model: str = "none", *, net: SgsNet | None = None,
                  cs: float = 0.17, kc: float = 21.0, g_scale: float = 1.0,
-                 tau_scale: float = 1.0, sgs_every: int = 1) -> None:
+                 tau_scale: float = 1.0) -> None:
         if cfg.dim != 3 or cfg.dtype != "float32":
             raise BackendUnavailable("LES: 3D fp32 flows")
         if model not in ("none", "smagorinsky", "net"):
@@ -254,35 +257,33 @@ class LESSolver(HipSolver):
         torch = self.torch
         L = cfg.n
         self.model, self.net, self.cs, self.kc = model, net, cs, kc
-        self.g_scale, self.tau_scale, self.sgs_every = g_scale, tau_scale, max(1, sgs_every)
+        self.g_scale, self.tau_scale = g_scale, tau_scale
         self.src = torch.zeros_like(self.buf.u0_rows)
         self.full = torch.zeros(3 * L ** 3, dtype=torch.complex64, device="cuda")
         self.grad = torch.zeros(6 * L ** 3, dtype=torch.complex64, device="cuda")
         self.tau = torch.zeros(3 * L ** 3, dtype=torch.complex64, device="cuda")
         self.X = torch.zeros((L ** 3, C), dtype=torch.bfloat16, device="cuda")
-        if model != "none":
-            _lib.check(self.lib.psx_set_source(self.ctx, _p(self.src)))
-        self._since = 0
+        if model == "none":
+            return
+        d = _lib.SgsDesc(model=_lib.SGS_MODELS[model], src=_p(self.src).value,
+                         full=_p(self.full).value, grad=_p(self.grad).value,
+                         tau=_p(self.tau).value, x=_p(self.X).value,
+                         g_scale=g_scale, tau_scale=tau_scale,
+                         smag=(cs * math.pi / kc) ** 2)
+        if model == "net":
+            assert net is not None and net.n == L
+            acts = net._buffers(1)
+            for i in range(NLAYER):
+                d.w[i] = _p(net.wb[i]).value
+                d.b[i] = _p(net.b[i]).value
+            for i in range(NLAYER - 1):
+                d.act[i] = _p(acts[i]).value
+            d.out = _p(acts[NLAYER - 1]).value
+        self._sgs = d  # the C context borrows the pointers; the tensors live as long as self
+        _lib.check(self.lib.psx_set_sgs(self.ctx, ctypes.byref(d)))
 
     def update_source(self) -> None:
-        """Subgrid-stress divergence of the current state into the source buffer."""
-        L = self.n
-        les_op("expand", self.buf.u0_rows, self.full, n=3, L=L, pitch=self.buf.pitch)
-        les_op("gradpack", self.full, self.grad, n=6, L=L, scale=1.0 / L ** 3)
-        fft3(self.grad, L, 6, inverse=True)
-        if self.model == "smagorinsky":
-            delta = math.pi / self.kc
-            les_op("smagorinsky", self.grad, self.tau, L=L, scale=(self.cs * delta) ** 2)
-        else:
-            les_op("features", self.grad, self.X, L=L, scale=self.g_scale)
-            y = self.net.forward(self.X, 1)  # type: ignore[union-attr]
-            les_op("tau_from_net", None, self.tau, L=L, scale=self.tau_scale, in2=y)
-        fft3(self.tau, L, 3, inverse=False)
-        les_op("source", self.tau, self.src, L=L, pitch=self.buf.pitch)
-
-    def _step(self, dt: float) -> None:
+        """Subgrid-stress divergence of the current state into the source buffer (the same
+        evaluation every step starts with)."""
         if self.model != "none":
-            if self._since % self.sgs_every == 0:
-                self.update_source()
-            self._since += 1
-        super()._step(dt)
+            _lib.check(self.lib.psx_sgs_update(self.ctx, _lib.stream_ptr()))

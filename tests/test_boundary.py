@@ -82,6 +82,11 @@ def test_planner_picks_compiled_variants(n, dim, form, fp64):
                  (lay.e_outer, lay.t_outer), (lay.e_mid, lay.t_mid)):
         if e:
             assert e in ok and 1 <= t <= 512
+    # the line kernel's work-groups hold whole rows and cover the n^(d-1) rows exactly: a
+    # tile that does not divide them would leave the last rows untransformed
+    assert (lay.t_line * lay.e_line) % n == 0
+    row_tile = lay.t_line * lay.e_line // n
+    assert (n ** (dim - 1)) % row_tile == 0, (n, dim, row_tile)
 
 
 def test_registry_surface():

@@ -265,3 +265,35 @@ def test_training_fits_a_teacher_network():
     print(f"  losses {['%.3e' % x for x in losses]}", flush=True)
     assert all(math.isfinite(x) for x in losses)
     assert losses[-1] < 0.5 * losses[0] and losses[-1] < losses[1]
+
+
+def test_attached_network_step_equals_explicit_source():
+    """A step of the LES with the network attached (psx_set_sgs: the C step evaluates the
+    stress first) equals the explicit evaluation followed by a plain step that adds the same
+    source, bit for bit; and the device controller's captured chunks equal its eager ones."""
+    L = 64
+    cfg = Config(n=L, dim=3, nu=2e-3, dtype="float32")
+    net = SgsNet(seed=21)
+    st = _state3(L, 6)
+    a = LESSolver(cfg, "net", net=net, g_scale=0.5, tau_scale=1e-3)
+    a.set_state(st)
+    a.step(0.01)
+    b = LESSolver(cfg, "net", net=net, g_scale=0.5, tau_scale=1e-3)
+    b.set_state(st)
+    b.update_source()
+    plain = get("hip/fused").make(cfg)
+    plain.set_state(st)
+    _lib.check(plain.lib.psx_set_source(plain.ctx, b.src.data_ptr()))
+    plain.step(0.01)
+    assert np.array_equal(a.get_state(), plain.get_state())
+    assert float(torch.abs(b.src).max()) > 0
+    runs = []
+    for use_graph in (False, True):
+        s = LESSolver(cfg, "net", net=net, g_scale=0.5, tau_scale=1e-3)
+        s.set_state(st)
+        r = s.run_device(math.inf, cfl=0.5, dt_max=0.05, cfl_every=10, kmax=25, chunk=5,
+                         use_graph=use_graph)
+        runs.append((r["dt"], s.get_state()))
+    (da, sa), (db, sb) = runs
+    print(f"  dt {da[0]:.4e} .. {da[-1]:.4e}", flush=True)
+    assert da == db and np.array_equal(sa, sb) and np.isfinite(sa).all()

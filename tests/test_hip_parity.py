@@ -71,7 +71,9 @@ def test_tgv2d_exact(name):
     assert O.rel_err(s.get_state(), w0 * math.exp(-2 * nu * 1.0)) < 1e-13
 
 
-@pytest.mark.parametrize("n,dtype,steps", [(96, "float64", 3), (128, "float32", 3),
+@pytest.mark.parametrize("n,dtype,steps", [(12, "float64", 3), (24, "float32", 3),
+                                           (48, "float64", 3), (96, "float64", 3),
+                                           (128, "float32", 3),
                                            (256, "float64", 2), (384, "float32", 2),
                                            (512, "float32", 2), (1024, "float32", 2),
                                            (1024, "float64", 2), (2048, "float32", 1),
