@@ -166,8 +166,10 @@ def ensemble_bench(a: argparse.Namespace) -> int:
     tune_s = time.perf_counter() - t0
     ens = EnsembleSolver(cfgs, tuned["variants"], wg_per_cu=tuned["wg_per_cu"])
     ens.set_states(states)
+    # depth 1: the host enqueues exactly the chunks that run (no no-op chunks past the end,
+    # whose launches would dilute the per-launch averages)
     run_kw = dict(cfl=0.5, dt_max=0.05, cfl_every=10, target_power=1e-3, chunk=10,
-                  record_start=False)
+                  record_start=False, depth=1)
     ens.run_device(math.inf, kmax=a.warmup, use_graph=False, **run_kw)
     torch.cuda.synchronize()
     lib, st = ens.lib, _lib.stream_ptr()
@@ -280,7 +282,9 @@ def tgv3d_bench(a: argparse.Namespace) -> int:
     s = get("hip/fused").make(cfg)
     s.set_state(w0)
     lay = s.buf.layout
-    run_kw = dict(cfl=0.5, dt_max=0.05, cfl_every=10, chunk=10, record_start=False)
+    # depth 1: the host enqueues exactly the chunks that run (no no-op chunks past the end,
+    # whose launches would dilute the per-launch averages)
+    run_kw = dict(cfl=0.5, dt_max=0.05, cfl_every=10, chunk=10, record_start=False, depth=1)
     s.run_device(math.inf, kmax=a.warmup, use_graph=False, **run_kw)
     torch.cuda.synchronize()
     lib, ctx, st = s.lib, s.ctx, _lib.stream_ptr()
@@ -392,7 +396,9 @@ def main(argv: list[str] | None = None) -> int:
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t_setup
     d = s.diag
-    run_kw = dict(cfl=0.5, dt_max=0.05, cfl_every=10, chunk=10, record_start=False)
+    # depth 1: the host enqueues exactly the chunks that run (no no-op chunks past the end,
+    # whose launches would dilute the per-launch averages)
+    run_kw = dict(cfl=0.5, dt_max=0.05, cfl_every=10, chunk=10, record_start=False, depth=1)
 
     # warm-up: the same controller loop (captures nothing; eager chunks)
     s.run_device(math.inf, kmax=a.warmup, use_graph=False, **run_kw)

@@ -156,6 +156,15 @@ def smagorinsky(g: np.ndarray, csd2: float) -> np.ndarray:
     return np.stack([-2.0 * csd2 * mag * S[i, j] for i, j in pairs])
 
 
+def clip_backscatter(tau: np.ndarray, g: np.ndarray, keep: float = 0.0) -> np.ndarray:
+    """Stress (order 11, 22, 33, 12, 13, 23) times ``keep`` wherever -tau_ij S_ij < 0, S the
+    strain rate of the gradient g[i][j] = d u_i / d x_j (no reference counterpart)."""
+    S = 0.5 * (g + np.swapaxes(g, 0, 1))
+    pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+    ts = sum((1.0 if i == j else 2.0) * tau[c] * S[i, j] for c, (i, j) in enumerate(pairs))
+    return np.where(ts > 0, keep * tau, tau)
+
+
 def stress_source(tau: np.ndarray, L: int) -> np.ndarray:
     """S_i = -i k_j tau_ij (rfftn of the physical stress), Leray-projected, 2/3-masked."""
     th = np.fft.rfftn(tau, axes=(1, 2, 3))

@@ -43,7 +43,8 @@ class RunState(ctypes.Structure):
 
 class LesArgs(ctypes.Structure):
     _fields_ = [("inp", c_void_p), ("in2", c_void_p), ("out", c_void_p), ("n", c_longlong),
-                ("L", c_int), ("L2", c_int), ("pitch", c_int), ("scale", c_double)]
+                ("L", c_int), ("L2", c_int), ("pitch", c_int), ("scale", c_double),
+                ("scale2", c_double)]
 
 
 class ConvArgs(ctypes.Structure):
@@ -62,7 +63,7 @@ class SgsDesc(ctypes.Structure):
     _fields_ = [("model", c_int), ("src", c_void_p), ("full", c_void_p), ("grad", c_void_p),
                 ("tau", c_void_p), ("x", c_void_p), ("w", c_void_p * 6), ("b", c_void_p * 6),
                 ("act", c_void_p * 5), ("out", c_void_p), ("g_scale", c_double),
-                ("tau_scale", c_double), ("smag", c_double)]
+                ("tau_scale", c_double), ("smag", c_double), ("backscatter", c_double)]
 
 
 SGS_MODELS = {"smagorinsky": 1, "net": 2}  # PSX_SGS_*

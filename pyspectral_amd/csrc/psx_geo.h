@@ -234,8 +234,10 @@ constexpr Geo3D geo3d(int n) {
   g.T = T;
   g.ldw = (g.jm + T - 1) / T * T;
   const bool r3 = n % 3 == 0;
+  // radix-3 sizes: 12 elements per thread (24 would need more than 256 VGPRs for the
+  // line kernel's several live sequences and drop it to one wave per SIMD)
   {
-    const int E = r3 ? 24 : 16;
+    const int E = r3 ? 12 : 16;
     int t = 1;
     while (n * t / E < 128 && (n * n) % (t * 2) == 0) t *= 2;  // t divides the n^2 rows
     const int nt = n * t / E;
@@ -253,8 +255,10 @@ constexpr Geo3D geo3d(int n) {
     return pl.np > 0 && nt >= 1 && nt <= 512 && (n * T) % E == 0;
   };
   if (n * T * 8 > 65536) return g;
-  if (!cols(r3 ? 24 : 16, g.yE, g.yNT, g.yp)) return g;
-  if (!cols(r3 ? 12 : 16, g.xE, g.xNT, g.xp)) return g;
+  if (!cols(r3 ? 12 : 16, g.yE, g.yNT, g.yp)) return g;
+  // the x pass keeps three accumulated components, the state and the transform in registers:
+  // 8 elements per thread from 128 up (16 spilled past 256 VGPRs)
+  if (!cols(r3 ? 12 : (n >= 128 ? 8 : 16), g.xE, g.xNT, g.xp)) return g;
   g.ok = true;
   return g;
 }

@@ -288,15 +288,32 @@ __global__ void k_les_smagorinsky(const cx<float>* __restrict__ ph, long long L3
 }
 
 // Network output (fp32, point-major, channels t11 t22 t33 t12 t13 t23 first) times scale as
-// stress pairs.
-__global__ void k_les_tau_from_net(const float* __restrict__ Y, long long L3, float scale,
+// stress pairs.  With the physical gradient pairs ph (else null), backscatter damping: where
+// the stress would move energy from the subgrid to the resolved scales, -tau_ij S_ij < 0
+// (S the resolved strain rate), it is multiplied by bs (0: clipped).
+__global__ void k_les_tau_from_net(const float* __restrict__ Y, const cx<float>* __restrict__ ph,
+                                   long long L3, float scale, float bs,
                                    cx<float>* __restrict__ tau) {
   const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= L3) return;
   const float* y = Y + r * kCnnC;
-  tau[r] = mkc(scale * y[0], scale * y[1]);
-  tau[L3 + r] = mkc(scale * y[2], scale * y[3]);
-  tau[2 * L3 + r] = mkc(scale * y[4], scale * y[5]);
+  float t[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) t[c] = scale * y[c];
+  if (ph) {
+    const float g00 = ph[L3 + r].y, g01 = ph[2 * L3 + r].x, g02 = ph[2 * L3 + r].y;
+    const float g10 = ph[3 * L3 + r].x, g11 = ph[3 * L3 + r].y, g12 = ph[4 * L3 + r].x;
+    const float g20 = ph[4 * L3 + r].y, g21 = ph[5 * L3 + r].x, g22 = ph[5 * L3 + r].y;
+    const float ts = t[0] * g00 + t[1] * g11 + t[2] * g22 +
+                     t[3] * (g01 + g10) + t[4] * (g02 + g20) + t[5] * (g12 + g21);
+    if (ts > 0.f) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) t[c] *= bs;
+    }
+  }
+  tau[r] = mkc(t[0], t[1]);
+  tau[L3 + r] = mkc(t[2], t[3]);
+  tau[2 * L3 + r] = mkc(t[4], t[5]);
 }
 
 // Subgrid-stress target tau_ij = filt(u_i u_j) - filt(u_i) filt(u_j) in physical space from
@@ -417,7 +434,8 @@ hipError_t launch_les(int which, const LesArgs& a, hipStream_t st) {
       break;
     case kLesTauFromNet:
       hipLaunchKernelGGL(k_les_tau_from_net, grid1(L3), dim3(256), 0, st,
-                         static_cast<const float*>(a.in2), L3, a.scale, a.cout);
+                         static_cast<const float*>(a.in2), a.cin, L3, a.scale, a.scale2,
+                         a.cout);
       break;
     case kLesTarget:
       hipLaunchKernelGGL(k_les_target, grid1(L3), dim3(256), 0, st, a.cin,

@@ -22,7 +22,8 @@ enum : int {
   kLesPairUp = 4,      // cin: 2 n full spectra, scale -> cout: n pairs
   kLesFeatures = 5,    // cin: gradient pairs (physical), scale = 1/g -> out: bf16 [L^3][32]
   kLesSmagorinsky = 6, // cin: gradient pairs, scale = (Cs D)^2 -> cout: 3 stress pairs
-  kLesTauFromNet = 7,  // in2: fp32 [L^3][32] network output, scale -> cout: 3 stress pairs
+  kLesTauFromNet = 7,  // in2: fp32 [L^3][32] network output, scale -> cout: 3 stress pairs;
+                       //   cin (or null): physical gradient pairs -> backscatter times scale2
   kLesTarget = 8,      // cin: filtered product pairs, in2: velocity pairs -> out: fp32 [L^3][8]
   kLesSource = 9,      // cin: stress pair spectra, pitch -> cout: source in the state layout
   kLesSumSq = 10,      // in2: fp32 values, n = count, pitch = width, L2 = channels counted
@@ -37,6 +38,7 @@ struct LesArgs {
   long long n = 0;
   int L = 0, L2 = 0, pitch = 0;
   float scale = 1.f;
+  float scale2 = 1.f;
 };
 
 hipError_t launch_les(int which, const LesArgs& a, hipStream_t st);

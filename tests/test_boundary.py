@@ -148,3 +148,29 @@ def test_random_phase_ic_spectrum_and_energy():
     model = k[sel] ** 4 * np.exp(-((k[sel] / k0) ** 2))
     ratio = e[sel] / model
     assert ratio.std() / ratio.mean() < 0.35  # shell sums follow k^4 exp(-(k/k0)^2)
+
+
+def test_checkpoint_round_trip(tmp_path):
+    """The reference's checkpoint format (validation.py:204-221): state, t, amp and the JSON
+    history survive a save/load into a fresh solver; the keys are the reference's."""
+    from pyspectral_amd import checkpoint
+    from pyspectral_amd.simulate import RunResult
+
+    cfg = Config(n=16, dim=2, nu=0.01, force_amp=0.2, force_klo=1, force_khi=3,
+                 dtype="float32")
+    a = O.OracleSolver(cfg)
+    a.set_state(O.to_spectral(O.random_2d(16, k0=3, seed=2), 2))
+    a.advance(3, 0.01)
+    a.set_forcing_amplitude(0.35)
+    res = RunResult(t=[0.0, 0.03], energy=[0.5, 0.49], enstrophy=[2.0, 1.9],
+                    dt=[0.01] * 3, steps=3, wall=0.1)
+    f = tmp_path / "ck.npz"
+    checkpoint.save(f, a, res, spectrum=np.arange(4.0))
+    z = np.load(f, allow_pickle=False)
+    assert set(z.files) == {"state", "t", "amp", "hist", "spectrum"}
+    assert z["state"].dtype == np.complex64
+    b = O.OracleSolver(cfg)
+    r2, z2 = checkpoint.load(f, b)
+    assert np.array_equal(b.get_state(), a.get_state()) and b.t == a.t
+    assert b.force_amp == 0.35 and r2.as_dict() == res.as_dict()
+    assert np.array_equal(z2["spectrum"], np.arange(4.0))

@@ -297,3 +297,36 @@ def test_attached_network_step_equals_explicit_source():
     (da, sa), (db, sb) = runs
     print(f"  dt {da[0]:.4e} .. {da[-1]:.4e}", flush=True)
     assert da == db and np.array_equal(sa, sb) and np.isfinite(sa).all()
+
+
+def test_network_stress_backscatter_clipping_matches_oracle():
+    """PSX_LES_TAU_FROM_NET with the gradient pairs multiplies the stress by the backscatter
+    factor exactly where the oracle's clip_backscatter does (-tau_ij S_ij < 0); without them
+    it only scales."""
+    L = 64
+    L3 = L ** 3
+    rng = np.random.default_rng(31)
+    g = rng.normal(size=(3, 3, L3)).astype(np.float32)
+    vel = rng.normal(size=(3, L3)).astype(np.float32)
+    comps = [vel[0], vel[1], vel[2]] + [g[i, j] for i in range(3) for j in range(3)]
+    pairs = np.stack([comps[2 * p] + 1j * comps[2 * p + 1] for p in range(6)])
+    Y = rng.normal(size=(L3, C)).astype(np.float32)
+    yc = torch.from_numpy(Y).cuda()
+    pc = torch.from_numpy(pairs.astype(np.complex64).reshape(-1)).cuda()
+    t6 = (0.5 * Y[:, :6].T).astype(np.float64)
+    ref = LO.clip_backscatter(t6, g.astype(np.float64))
+    half = LO.clip_backscatter(t6, g.astype(np.float64), keep=0.5)
+    for grad, keep, want in ((pc, 0.0, ref), (pc, 0.5, half), (None, 0.0, t6)):
+        tau = torch.zeros(3 * L3, dtype=torch.complex64, device="cuda")
+        les_op("tau_from_net", grad, tau, L=L, scale=0.5, scale2=keep, in2=yc)
+        tp = tau.cpu().numpy().reshape(3, L3)
+        got = np.stack([tp[0].real, tp[0].imag, tp[1].real, tp[1].imag, tp[2].real, tp[2].imag])
+        # points whose transfer is within fp32 round-off of zero may fall either way
+        S = 0.5 * (g + np.swapaxes(g, 0, 1)).astype(np.float64)
+        ts = sum((1.0 if i == j else 2.0) * t6[c] * S[i, j]
+                 for c, (i, j) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]))
+        sure = np.abs(ts) > 1e-4 * np.abs(ts).max()
+        assert np.abs(got - want)[:, sure].max() < 1e-6 * np.abs(want).max()
+    frac = float(np.mean(np.all(ref == 0, axis=0)))
+    print(f"  clipped fraction {frac:.3f}", flush=True)
+    assert 0.3 < frac < 0.7

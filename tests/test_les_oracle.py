@@ -109,6 +109,26 @@ def test_smagorinsky_shear_flow():
         np.testing.assert_allclose(tau[c], 0, atol=1e-12)
 
 
+def test_backscatter_clipping():
+    """Smagorinsky stress only drains energy (-tau:S = 2 (Cs D)^2 |S|^3 >= 0) and is kept; its
+    negative is pure backscatter and is zeroed; a mixed field keeps exactly the points where
+    the transfer is forward.  Shear flow u = (sin y, 0, 0): -tau:S = 2 c |S|^3."""
+    L = 16
+    rng = np.random.default_rng(3)
+    u = rng.normal(size=(3, L, L, L))
+    g = LO.gradient(np.fft.rfftn(u, axes=(1, 2, 3)), L)
+    smag = LO.smagorinsky(g, 0.02)
+    np.testing.assert_array_equal(LO.clip_backscatter(smag, g), smag)
+    assert np.all(LO.clip_backscatter(-smag, g) == 0)
+    sign = np.where(rng.random((L, L, L)) < 0.5, 1.0, -1.0)
+    mixed = smag * sign
+    got = LO.clip_backscatter(mixed, g)
+    keep = (sign > 0) | np.all(smag == 0, axis=0)
+    np.testing.assert_array_equal(got, np.where(keep, mixed, 0.0))
+    np.testing.assert_array_equal(LO.clip_backscatter(mixed, g, keep=0.25),
+                                  np.where(keep, mixed, 0.25 * mixed))
+
+
 def test_filter_transfer_and_truncation():
     L, Lc = 32, 16
     x = 2 * np.pi * np.arange(L) / L
