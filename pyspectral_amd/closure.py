@@ -162,13 +162,13 @@ def _train(X: Any, T: Any, n_les: int, n_val: int, train_steps: int, batch: int,
 
 
 def _save_model(path: str, net: SgsNet, dns: dict[str, Any], filt_state: np.ndarray,
-                training: dict[str, Any]) -> None:
+                training: dict[str, Any], config: dict[str, Any]) -> None:
     st = net.state()
-    np.savez(path, w=st["w"], b=st["b"], filt_state=filt_state,
-             dns=json.dumps(dns), training=json.dumps(training))
+    np.savez(path, w=st["w"], b=st["b"], filt_state=filt_state, dns=json.dumps(dns),
+             training=json.dumps(training), config=json.dumps(config))
 
 
-def _load_model(path: str, n_les: int) -> tuple[SgsNet, dict, np.ndarray, dict]:
+def _load_model(path: str, n_les: int) -> tuple[SgsNet, dict, np.ndarray, dict, dict]:
     import torch
 
     z = np.load(path, allow_pickle=False)
@@ -176,7 +176,9 @@ def _load_model(path: str, n_les: int) -> tuple[SgsNet, dict, np.ndarray, dict]:
     net.w.copy_(torch.from_numpy(z["w"]))
     net.b.copy_(torch.from_numpy(z["b"]))
     net._adam(lr=0.0)  # bf16 copies of the loaded weights
-    return net, json.loads(str(z["dns"])), z["filt_state"], json.loads(str(z["training"]))
+    config = json.loads(str(z["config"])) if "config" in z.files else {}
+    return (net, json.loads(str(z["dns"])), z["filt_state"], json.loads(str(z["training"])),
+            config)
 
 
 def closure_study(*, n_dns: int = 256, n_les: int = 64, kc: float = 21.0, nu: float = 1.6e-3,
@@ -197,7 +199,12 @@ def closure_study(*, n_dns: int = 256, n_les: int = 64, kc: float = 21.0, nu: fl
     run_kw = dict(cfl=0.5, dt_max=0.05, target_power=power, record_start=False)
 
     if load_model:
-        net, out["dns"], filt_state, out["training"] = _load_model(load_model, n_les)
+        net, out["dns"], filt_state, out["training"], model_cfg = _load_model(load_model, n_les)
+        # the DNS and training settings are the saved run's, not this call's defaults
+        out["config"] = {**{k: out["config"][k] for k in ("t_les", "t_avg", "backscatter",
+                                                          "load_model", "kc", "nu", "power",
+                                                          "band", "n_les")},
+                         "model_config": model_cfg}
         log(f"loaded the trained network and the filtered-DNS data from {load_model}")
     else:
         d = _dns_and_snapshots(cfg_dns, n_les, kc, nu, t_spin, n_snap, snap_every, t_cost,
@@ -208,7 +215,8 @@ def closure_study(*, n_dns: int = 256, n_les: int = 64, kc: float = 21.0, nu: fl
         del d
         torch.cuda.empty_cache()
         if save_model:
-            _save_model(save_model, net, out["dns"], filt_state, out["training"])
+            _save_model(save_model, net, out["dns"], filt_state, out["training"],
+                        out["config"])
     g_rms, tau_rms = out["dns"]["scales"]["g_rms"], out["dns"]["scales"]["tau_rms"]
     ek_f = np.array(out["dns"]["filtered_Ek"])
     dns_per_time = out["dns"]["gpu_s_per_unit_time"]

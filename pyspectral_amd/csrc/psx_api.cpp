@@ -742,6 +742,8 @@ int psx_set_sgs(psx_ctx* c, const psx_sgs_desc* d) {
       if (!d->w[i] || !d->b[i] || (i < 5 && !d->act[i]))
         return fail("psx_set_sgs: null network tensor (layer %d)", i);
   }
+  // tables the evaluation needs are made here, never inside a graph capture of a step
+  if (sgs_prepare(c->p.n)) return 1;
   const int r = psx_set_source(c, d->src);  // also drops the captured graphs
   if (r) return r;
   c->sgs = *d;

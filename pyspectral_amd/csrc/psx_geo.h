@@ -145,6 +145,14 @@ constexpr Geo2D geo2d(int n, bool fp64) {
     g.n2 = best;
     g.n1 = n / best;
     if (!cols(g.n2, g.specE, g.specNT, g.spec)) return g;
+    // fp32: 8 elements per thread in the inner kernel, which holds both fields' transforms,
+    // the state update and the next inputs (16 needed all 256 VGPRs: two waves per SIMD)
+    if (!fp64 && g.specE == 16) {
+      g.specE = 8;
+      g.spec = ct_plan(g.n2, 8);
+      g.specNT = g.n2 * T / 8;
+      if (g.spec.np == 0 || g.specNT > 512) return g;
+    }
     if (!cols(g.n1, g.outE, g.outNT, g.outer)) return g;
   }
   g.ok = true;
@@ -229,7 +237,10 @@ constexpr Geo3D geo3d(int n) {
   g.n = n;
   g.jm = ct_retained(n);
   g.sp = ct_state_pitch(n, false);
-  int T = 16;
+  // small power-of-two grids: 4-column tiles and short rows per thread, so that the column
+  // passes launch more work-groups than the GPU has CUs (64^3 with 16-column tiles: 128)
+  const bool small = n <= 64 && (n & (n - 1)) == 0;
+  int T = small ? 4 : 16;
   while (T > 1 && T / 2 >= g.jm) T /= 2;
   g.T = T;
   g.ldw = (g.jm + T - 1) / T * T;
@@ -237,7 +248,7 @@ constexpr Geo3D geo3d(int n) {
   // radix-3 sizes: 12 elements per thread (24 would need more than 256 VGPRs for the
   // line kernel's several live sequences and drop it to one wave per SIMD)
   {
-    const int E = r3 ? 12 : 16;
+    const int E = r3 ? 12 : (small ? 8 : 16);
     int t = 1;
     while (n * t / E < 128 && (n * n) % (t * 2) == 0) t *= 2;  // t divides the n^2 rows
     const int nt = n * t / E;
@@ -255,10 +266,10 @@ constexpr Geo3D geo3d(int n) {
     return pl.np > 0 && nt >= 1 && nt <= 512 && (n * T) % E == 0;
   };
   if (n * T * 8 > 65536) return g;
-  if (!cols(r3 ? 12 : 16, g.yE, g.yNT, g.yp)) return g;
+  if (!cols(r3 ? 12 : (small ? 4 : 16), g.yE, g.yNT, g.yp)) return g;
   // the x pass keeps three accumulated components, the state and the transform in registers:
   // 8 elements per thread from 128 up (16 spilled past 256 VGPRs)
-  if (!cols(r3 ? 12 : (n >= 128 ? 8 : 16), g.xE, g.xNT, g.xp)) return g;
+  if (!cols(r3 ? 12 : (small ? 4 : (n >= 128 ? 8 : 16)), g.xE, g.xNT, g.xp)) return g;
   g.ok = true;
   return g;
 }

@@ -202,16 +202,28 @@ hipError_t launch_conv(const ConvArgs& a, hipStream_t st) {
   return hipGetLastError();
 }
 
-// Sum of the per-work-group partials (fixed order), added into dw / db.
-__global__ void k_wgrad_reduce(const float* __restrict__ part, int ngroups, float* dw,
-                               float* db) {
+// Sum of the per-work-group partials, added into dw / db.  A work-group owns 64 consecutive
+// entries; its 4 waves sum the partials g = wave, wave + 4, ... in order, and the four wave
+// sums are added in a fixed order, so the result does not depend on scheduling.
+constexpr int kWgRedCols = 64;
+
+__global__ void __launch_bounds__(256) k_wgrad_reduce(const float* __restrict__ part,
+                                                      int ngroups, float* dw, float* db) {
   constexpr int NW = kCnnTaps * kCnnC * kCnnC;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= NW + kCnnC) return;
+  constexpr int NT = NW + kCnnC;
+  __shared__ float red[4][kWgRedCols];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = blockIdx.x * kWgRedCols + lane;
   float s = 0.f;
-  for (int g = 0; g < ngroups; ++g) s += part[(long long)g * (NW + kCnnC) + i];
-  if (i < NW) dw[i] += s;
-  else db[i - NW] += s;
+  if (i < NT)
+    for (int g = wave; g < ngroups; g += 4) s += part[(long long)g * NT + i];
+  red[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && i < NT) {
+    const float t = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+    if (i < NW) dw[i] += t;
+    else db[i - NW] += t;
+  }
 }
 
 // ------------------------------------------------------ weight gradient, n = 64, MFMA-only
@@ -360,27 +372,41 @@ hipError_t launch_conv_wgrad(const __bf16* x, const __bf16* dz, float* dw, float
   const int nbricks = nb * (n * n * n / kCvPts);
   hipLaunchKernelGGL(k_wgrad64, dim3(kWg64Grid), dim3(256), kCvLds, st, x, dz, nbricks, part);
   const int nw = kCnnTaps * kCnnC * kCnnC + kCnnC;
-  hipLaunchKernelGGL(k_wgrad_reduce, dim3((nw + 255) / 256), dim3(256), 0, st, part,
-                     kWgGroups, dw, db);
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3((nw + kWgRedCols - 1) / kWgRedCols), dim3(256), 0,
+                     st, part, kWgGroups, dw, db);
   return hipGetLastError();
 }
 
 // -------------------------------------------------------------------------------- loss
-__global__ void k_mse(const float* __restrict__ y, const float* __restrict__ t, int tstride,
-                      int nch, long long npts, float inv_count, __bf16* __restrict__ g,
-                      double* loss) {
+// One thread per point: the point's 32 outputs (128 B) and targets come in as float4 loads,
+// the 32 bf16 gradients go out as four 16-byte stores.
+__global__ void __launch_bounds__(256) k_mse(const float* __restrict__ y,
+                                             const float* __restrict__ t, int tstride, int nch,
+                                             long long npts, float inv_count,
+                                             __bf16* __restrict__ g, double* loss) {
   __shared__ double red[256];
   double a = 0.0;
   for (long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x; p < npts;
        p += (long long)gridDim.x * blockDim.x) {
-    for (int c = 0; c < kCnnC; ++c) {
-      float gv = 0.f;
-      if (c < nch) {
-        const float d = y[p * kCnnC + c] - t[p * tstride + c];
-        a += (double)d * d;
-        gv = 2.f * d * inv_count;
+    const float4* yp = reinterpret_cast<const float4*>(y + p * kCnnC);
+    const float* tp = t + p * tstride;
+#pragma unroll
+    for (int q = 0; q < kCnnC / 8; ++q) {
+      const float4 lo = yp[2 * q], hi = yp[2 * q + 1];
+      const float yv[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+      bf16x8 o;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int c = 8 * q + j;
+        float gv = 0.f;
+        if (c < nch) {
+          const float d = yv[j] - tp[c];
+          a += (double)d * d;
+          gv = 2.f * d * inv_count;
+        }
+        o[j] = (__bf16)gv;
       }
-      g[p * kCnnC + c] = (__bf16)gv;
+      *reinterpret_cast<bf16x8*>(g + p * kCnnC + 8 * q) = o;
     }
   }
   red[threadIdx.x] = a;

@@ -40,6 +40,11 @@ const cx<float>* c2c_twiddles(int L) {
 
 namespace psx {
 
+int sgs_prepare(int L) {
+  if (!c2c_twiddles(L)) return fail("sgs: twiddle table allocation failed");
+  return 0;
+}
+
 int enqueue_sgs(const psx_sgs_desc& d, const cx<float>* u0, int pitch, int L, hipStream_t st) {
   const cx<float>* tw = c2c_twiddles(L);
   if (!tw) return fail("sgs: twiddle table allocation failed");

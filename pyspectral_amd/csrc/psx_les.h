@@ -93,5 +93,8 @@ hipError_t launch_adam(const AdamArgs& a, hipStream_t st);
 // Subgrid-stress source of the LES model d (psx_set_sgs) from the state u0 (3 components of
 // L^2 rows, row pitch `pitch`) into d.src; 0 or a psx_last_error code.
 int enqueue_sgs(const psx_sgs_desc& d, const cx<float>* u0, int pitch, int L, hipStream_t st);
+// Allocate what enqueue_sgs uses for grid L (the c2c twiddle tables) now, so that a step
+// captured into a graph later allocates nothing; 0 or a psx_last_error code.
+int sgs_prepare(int L);
 
 }  // namespace psx
