@@ -306,6 +306,16 @@ typedef struct psx_adam_args {
 } psx_adam_args;
 int psx_cnn_adam(const psx_adam_args* a, void* stream);
 
+/* ---- FFT engine test hooks (no counterpart in the reference, whose FFTs are cuFFT) -------
+ * psx_fft_lines: in-place complex fp32 FFT of nrows contiguous rows of length L with the plan,
+ * work-group shape and twiddle tables of the line kernel for that length (L = 64..8192 powers
+ * of two: the 2D kernels'; 96, 192, 384: the 3D kernels'); inverse = 1 for exp(+2 pi i k n / L),
+ * unnormalised.  psx_fft_twiddles: that plan's radices (forward or reversed order) and its
+ * per-pass twiddle table in that order, as float pairs (pass q of radix R after Ns points holds
+ * exp(-2 pi i m k / (Ns R)) at (m - 1) Ns + k, m < R, k < Ns; passes in order, pass 0 none). */
+int psx_fft_lines(void* data, int L, int nrows, int inverse, void* stream);
+int psx_fft_twiddles(int L, int inverse, float* out, int cap, int* radices, int* npasses);
+
 /* Message of the last failure on this thread. */
 const char* psx_last_error(void);
 

@@ -121,6 +121,8 @@ SIGNATURES = {
     "psx_cnn_mse": (c_int, [c_void_p, c_void_p, c_int, c_int, c_longlong, c_void_p, c_void_p,
                             c_void_p]),
     "psx_cnn_adam": (c_int, [POINTER(AdamArgs), c_void_p]),
+    "psx_fft_lines": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
+    "psx_fft_twiddles": (c_int, [c_int, c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "psx_last_error": (c_char_p, []),
     "psx_device_info": (c_int, [c_char_p, c_int]),
 }

@@ -18,8 +18,8 @@ CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libpsx.so"
 SOURCES = ["psx_k2d.hip", "psx_k2d_ct.hip", "psx_k3d.hip", "psx_k3d_ct.hip", "psx_kutil.hip",
-           "psx_kdrv.hip", "psx_kens.hip", "psx_kles.hip", "psx_kcnn.hip", "psx_api.cpp",
-           "psx_ens.cpp", "psx_les.cpp"]
+           "psx_kdrv.hip", "psx_kens.hip", "psx_kles.hip", "psx_kcnn.hip", "psx_kfft.hip",
+           "psx_api.cpp", "psx_ens.cpp", "psx_les.cpp"]
 ARCH = os.environ.get("PSX_OFFLOAD_ARCH", "gfx950")
 
 # Packed-FP32 instructions stay disabled for every device compile.
