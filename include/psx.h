@@ -235,9 +235,11 @@ typedef struct psx_sgs_desc {
   float* out;         /* network output, fp32 [L^3][32] (channels 0..5 used) */
   double g_scale;     /* input scale: 1 / velocity-gradient rms */
   double tau_scale;   /* output scale: stress rms */
-  double smag;        /* Smagorinsky (Cs Delta)^2 */
-  double backscatter; /* network: factor on the stress where -tau_ij S_ij < 0, i.e. where it
-                         would move energy to the resolved scales (1 keeps it, 0 clips it) */
+  double smag;        /* Smagorinsky (Cs Delta)^2; with PSX_SGS_NET, > 0 adds the Smagorinsky
+                         stress to the network's (a mixed model), 0 leaves it out */
+  double backscatter; /* network: factor in [0, 2] on the stress where -tau_ij S_ij < 0, i.e.
+                         where it would move energy to the resolved scales (1 keeps it, 0 clips
+                         it, > 1 amplifies it) */
 } psx_sgs_desc;
 int psx_set_sgs(psx_ctx* c, const psx_sgs_desc* d);
 int psx_sgs_update(psx_ctx* c, void* stream);
@@ -256,6 +258,7 @@ typedef struct psx_les_args {
   int L, L2, pitch; /* grid size, second grid size, row pitch or flag */
   double scale;
   double scale2;    /* second factor (PSX_LES_TAU_FROM_NET: backscatter factor) */
+  double scale3;    /* third factor (PSX_LES_TAU_FROM_NET: Smagorinsky (Cs D)^2 added, 0: none) */
 } psx_les_args;
 #define PSX_LES_EXPAND 0     /* state (pitch) with n components -> full L^3 spectra */
 #define PSX_LES_GRADPACK 1   /* 3 full spectra -> n = 6 pairs (u,v)(w,ux)(uy,uz)(vx,vy)(vz,wx)(wy,wz), or n = 1: (u,v)(w,0); times scale */
@@ -264,7 +267,7 @@ typedef struct psx_les_args {
 #define PSX_LES_PAIRUP 4     /* 2n full spectra -> n pairs, times scale */
 #define PSX_LES_FEATURES 5   /* physical gradient pairs -> bf16 [L^3][32] model input, times scale */
 #define PSX_LES_SMAGORINSKY 6 /* physical gradient pairs -> stress pairs, scale = (Cs D)^2 */
-#define PSX_LES_TAU_FROM_NET 7 /* in2: fp32 [L^3][32] model output -> stress pairs, times scale; with in = physical gradient pairs, the stress is multiplied by scale2 where -tau_ij S_ij < 0 (backscatter; 0 clips it) */
+#define PSX_LES_TAU_FROM_NET 7 /* in2: fp32 [L^3][32] model output -> stress pairs, times scale; with in = physical gradient pairs, the stress is multiplied by scale2 where -tau_ij S_ij < 0 (backscatter; 0 clips it), and for scale3 > 0 the Smagorinsky stress with (Cs D)^2 = scale3 is added */
 #define PSX_LES_TARGET 8     /* filtered product pairs, in2: velocity pairs -> fp32 [L^3][8] stress, times scale */
 #define PSX_LES_SOURCE 9     /* stress pair spectra -> projected -i k_j tau_ij, state layout (pitch) */
 #define PSX_LES_SUMSQ 10     /* in2: n fp32 values in rows of `pitch`, first L2 of each -> += sum of squares (double) */

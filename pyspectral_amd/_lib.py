@@ -44,7 +44,7 @@ class RunState(ctypes.Structure):
 class LesArgs(ctypes.Structure):
     _fields_ = [("inp", c_void_p), ("in2", c_void_p), ("out", c_void_p), ("n", c_longlong),
                 ("L", c_int), ("L2", c_int), ("pitch", c_int), ("scale", c_double),
-                ("scale2", c_double)]
+                ("scale2", c_double), ("scale3", c_double)]
 
 
 class ConvArgs(ctypes.Structure):

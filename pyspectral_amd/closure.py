@@ -8,7 +8,8 @@ P = 0.1, nu = 1.6e-3 (Re_lambda about 80); snapshots box-filtered (k_c = 21) ont
 LES grid; six 3x3x3 convolutions of 32 channels trained with Adam (batch 8, cosine
 learning-rate decay) on the GPU; 64^3 LES runs from the filtered DNS field with the same
 forcing, averaged over all but its first 3 time units: no model, Smagorinsky
-(Cs = 0.17), the network with backscatter clipping and the network without it.  Reported:
+(Cs = 0.17), the network with backscatter clipping and the network without it, and mixed
+models (the network plus a Smagorinsky stress of a small constant).  Reported:
 the time-averaged E(k) error of each LES against the time-averaged filtered DNS spectrum for
 1 <= k <= k_c, the resolved dissipation 2 nu Z, the DNS's Re_lambda, k_max eta and
 large-eddy time, and GPU time per unit simulated time of the DNS and each LES.  Every kernel
@@ -187,6 +188,7 @@ def closure_study(*, n_dns: int = 256, n_les: int = 64, kc: float = 21.0, nu: fl
                   n_val: int = 16, train_steps: int = 2000, batch: int = 8, lr: float = 1e-3,
                   t_les: float = 30.0, t_avg: float = 27.0, t_cost: float = 5.0, seed: int = 0,
                   backscatter: tuple[float, ...] = (1.0, 0.5, 0.0),
+                  mixed: tuple[tuple[float, ...], ...] = ((1.0, 0.04),),
                   save_model: str | None = None, load_model: str | None = None,
                   log: Any = print) -> dict[str, Any]:
     import torch
@@ -201,7 +203,7 @@ def closure_study(*, n_dns: int = 256, n_les: int = 64, kc: float = 21.0, nu: fl
     if load_model:
         net, out["dns"], filt_state, out["training"], model_cfg = _load_model(load_model, n_les)
         # the DNS and training settings are the saved run's, not this call's defaults
-        out["config"] = {**{k: out["config"][k] for k in ("t_les", "t_avg", "backscatter",
+        out["config"] = {**{k: out["config"][k] for k in ("t_les", "t_avg", "backscatter", "mixed",
                                                           "load_model", "kc", "nu", "power",
                                                           "band", "n_les")},
                          "model_config": model_cfg}
@@ -226,14 +228,21 @@ def closure_study(*, n_dns: int = 256, n_les: int = 64, kc: float = 21.0, nu: fl
     # simulated time (as the DNS is timed), and one run from the same state that stops at
     # the spectrum sample times of the averaging window (every 0.25), landing on them as
     # simulate.run lands on its callback times.
-    models = [("none", "none", 1.0), ("smagorinsky", "smagorinsky", 1.0)]
-    models += [(f"net_bs{b:g}", "net", b) for b in backscatter]
+    models = [("none", "none", 1.0, 0.0, 1.0), ("smagorinsky", "smagorinsky", 1.0, 0.0, 1.0)]
+    models += [(f"net_bs{b:g}", "net", b, 0.0, 1.0) for b in backscatter]
+    # mixed models: the network (backscatter factor b, its whole stress times g) plus a
+    # Smagorinsky stress of constant c
+    for m in mixed:
+        b, c, g = (*m, 1.0)[:3]
+        models.append((f"net_bs{b:g}_cs{c:g}" + (f"_x{g:g}" if g != 1.0 else ""), "net", b, c,
+                       g))
     res = {}
     t_samples = np.arange(t_les - t_avg, t_les + 1e-9, 0.25)
-    for key, model, bs in models:
+    for key, model, bs, mix_cs, gain in models:
         def make() -> LESSolver:
             s = LESSolver(cfg_les, model, net=net if model == "net" else None, kc=kc,
-                          g_scale=1.0 / g_rms, tau_scale=tau_rms, backscatter=bs)
+                          g_scale=1.0 / g_rms, tau_scale=tau_rms * gain, backscatter=bs,
+                          mix_cs=mix_cs)
             s.set_state(filt_state)
             return s
 
@@ -257,7 +266,8 @@ def closure_study(*, n_dns: int = 256, n_les: int = 64, kc: float = 21.0, nu: fl
         ek /= len(t_samples)
         k = np.arange(1, int(kc) + 1)
         err = (ek[k] - ek_f[k]) / ek_f[k]
-        r = {"model": model, "backscatter": bs, "steps": steps, "wall_s": wall,
+        r = {"model": model, "backscatter": bs, "mix_cs": mix_cs, "net_gain": gain,
+             "steps": steps, "wall_s": wall,
              "gpu_s_per_unit_time": wall / t_les,
              "speedup_vs_dns": dns_per_time / (wall / t_les), "Ek": ek.tolist(),
              "Ek_rel_err": err.tolist(), "Ek_rel_err_max": float(np.abs(err).max()),
@@ -287,6 +297,9 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--t-avg", type=float, default=None)
     ap.add_argument("--backscatter", default=None,
                     help="comma-separated backscatter factors of the network LES runs")
+    ap.add_argument("--mixed", default=None,
+                    help="comma-separated b:c[:g] triples: the network (backscatter factor b, "
+                         "stress times g, default 1) plus a Smagorinsky stress of constant c")
     ap.add_argument("--save-model", default=None, help="save the network and the DNS data")
     ap.add_argument("--load-model", default=None, help="skip DNS and training: load them")
     a = ap.parse_args(argv)
@@ -302,6 +315,8 @@ def main(argv: list[str] | None = None) -> int:
             kw[key] = val
     if a.backscatter:
         kw["backscatter"] = tuple(float(x) for x in a.backscatter.split(","))
+    if a.mixed:
+        kw["mixed"] = tuple(tuple(float(v) for v in x.split(":")) for x in a.mixed.split(","))
     res = closure_study(log=lambda m: print(m, flush=True), **kw)
     txt = json.dumps(res)
     if a.out:

@@ -733,8 +733,10 @@ int psx_set_sgs(psx_ctx* c, const psx_sgs_desc* d) {
   if (!c2c_supported(c->p.n)) return fail("psx_set_sgs: n=%d unsupported (64, 128, 256)", c->p.n);
   if (d->model != PSX_SGS_SMAGORINSKY && d->model != PSX_SGS_NET)
     return fail("psx_set_sgs: unknown model %d", d->model);
-  if (!(d->backscatter >= 0.0 && d->backscatter <= 1.0))
-    return fail("psx_set_sgs: backscatter factor %g outside [0, 1]", d->backscatter);
+  if (!(d->backscatter >= 0.0 && d->backscatter <= 2.0))
+    return fail("psx_set_sgs: backscatter factor %g outside [0, 2]", d->backscatter);
+  if (!(d->smag >= 0.0) || (d->model == PSX_SGS_SMAGORINSKY && d->smag == 0.0))
+    return fail("psx_set_sgs: Smagorinsky (Cs Delta)^2 = %g", d->smag);
   if (!d->src || !d->full || !d->grad || !d->tau) return fail("psx_set_sgs: null buffer");
   if (d->model == PSX_SGS_NET) {
     if (!d->x || !d->out) return fail("psx_set_sgs: null network buffer");

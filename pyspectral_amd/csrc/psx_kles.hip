@@ -288,11 +288,12 @@ __global__ void k_les_smagorinsky(const cx<float>* __restrict__ ph, long long L3
 }
 
 // Network output (fp32, point-major, channels t11 t22 t33 t12 t13 t23 first) times scale as
-// stress pairs.  With the physical gradient pairs ph (else null), backscatter damping: where
+// stress pairs.  With the physical gradient pairs ph (else null): backscatter damping, where
 // the stress would move energy from the subgrid to the resolved scales, -tau_ij S_ij < 0
-// (S the resolved strain rate), it is multiplied by bs (0: clipped).
+// (S the resolved strain rate), it is multiplied by bs (0: clipped); and, for csd2 > 0, the
+// Smagorinsky stress -2 csd2 |S| S_ij added to it (a mixed model; csd2 = (Cs Delta)^2).
 __global__ void k_les_tau_from_net(const float* __restrict__ Y, const cx<float>* __restrict__ ph,
-                                   long long L3, float scale, float bs,
+                                   long long L3, float scale, float bs, float csd2,
                                    cx<float>* __restrict__ tau) {
   const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= L3) return;
@@ -309,6 +310,18 @@ __global__ void k_les_tau_from_net(const float* __restrict__ Y, const cx<float>*
     if (ts > 0.f) {
 #pragma unroll
       for (int c = 0; c < 6; ++c) t[c] *= bs;
+    }
+    if (csd2 > 0.f) {
+      const float s12 = 0.5f * (g01 + g10), s13 = 0.5f * (g02 + g20), s23 = 0.5f * (g12 + g21);
+      const float ss = g00 * g00 + g11 * g11 + g22 * g22 +
+                       2.f * (s12 * s12 + s13 * s13 + s23 * s23);
+      const float c = -2.f * csd2 * sqrtf(2.f * ss);
+      t[0] += c * g00;
+      t[1] += c * g11;
+      t[2] += c * g22;
+      t[3] += c * s12;
+      t[4] += c * s13;
+      t[5] += c * s23;
     }
   }
   tau[r] = mkc(t[0], t[1]);
@@ -435,7 +448,7 @@ hipError_t launch_les(int which, const LesArgs& a, hipStream_t st) {
     case kLesTauFromNet:
       hipLaunchKernelGGL(k_les_tau_from_net, grid1(L3), dim3(256), 0, st,
                          static_cast<const float*>(a.in2), a.cin, L3, a.scale, a.scale2,
-                         a.cout);
+                         a.scale3, a.cout);
       break;
     case kLesTarget:
       hipLaunchKernelGGL(k_les_target, grid1(L3), dim3(256), 0, st, a.cin,

@@ -52,7 +52,7 @@ int enqueue_sgs(const psx_sgs_desc& d, const cx<float>* u0, int pitch, int L, hi
   auto* grad = static_cast<cx<float>*>(d.grad);
   auto* tau = static_cast<cx<float>*>(d.tau);
   auto op = [&](int which, const void* in, const void* in2, void* out, long long n, int pch,
-                double scale, double scale2 = 1.0) {
+                double scale, double scale2 = 1.0, double scale3 = 0.0) {
     LesArgs a;
     a.cin = static_cast<const cx<float>*>(in);
     a.in2 = in2;
@@ -63,6 +63,7 @@ int enqueue_sgs(const psx_sgs_desc& d, const cx<float>* u0, int pitch, int L, hi
     a.pitch = pch;
     a.scale = (float)scale;
     a.scale2 = (float)scale2;
+    a.scale3 = (float)scale3;
     return launch_les(which, a, st);
   };
   // velocity gradients of the state in physical space, six complex pairs
@@ -88,9 +89,10 @@ int enqueue_sgs(const psx_sgs_desc& d, const cx<float>* u0, int pitch, int L, hi
       }
       PSX_HIP(launch_conv(c, st));
     }
-    const bool bs = d.backscatter != 1.0;  // 1: the network's stress as it is
-    PSX_HIP(op(kLesTauFromNet, bs ? grad : nullptr, d.out, tau, 0, 0, d.tau_scale,
-               d.backscatter));
+    // 1 and no Smagorinsky part: the network's stress as it is
+    const bool pointwise = d.backscatter != 1.0 || d.smag > 0.0;
+    PSX_HIP(op(kLesTauFromNet, pointwise ? grad : nullptr, d.out, tau, 0, 0, d.tau_scale,
+               d.backscatter, d.smag > 0.0 ? d.smag : 0.0));
   }
   PSX_HIP(launch_c2c3(tau, tw, L, 3, false, st));
   PSX_HIP(op(kLesSource, tau, nullptr, d.src, 0, pitch, 1.0));
@@ -125,6 +127,7 @@ int psx_les_op(int which, const psx_les_args* a, void* stream) {
   x.pitch = a->pitch;
   x.scale = (float)a->scale;
   x.scale2 = (float)a->scale2;
+  x.scale3 = (float)a->scale3;
   PSX_HIP(launch_les(which, x, static_cast<hipStream_t>(stream)));
   return 0;
 }

@@ -23,7 +23,8 @@ enum : int {
   kLesFeatures = 5,    // cin: gradient pairs (physical), scale = 1/g -> out: bf16 [L^3][32]
   kLesSmagorinsky = 6, // cin: gradient pairs, scale = (Cs D)^2 -> cout: 3 stress pairs
   kLesTauFromNet = 7,  // in2: fp32 [L^3][32] network output, scale -> cout: 3 stress pairs;
-                       //   cin (or null): physical gradient pairs -> backscatter times scale2
+                       //   cin (or null): physical gradient pairs -> backscatter times scale2,
+                       //   plus the Smagorinsky stress for scale3 = (Cs D)^2 > 0
   kLesTarget = 8,      // cin: filtered product pairs, in2: velocity pairs -> out: fp32 [L^3][8]
   kLesSource = 9,      // cin: stress pair spectra, pitch -> cout: source in the state layout
   kLesSumSq = 10,      // in2: fp32 values, n = count, pitch = width, L2 = channels counted
@@ -39,6 +40,7 @@ struct LesArgs {
   int L = 0, L2 = 0, pitch = 0;
   float scale = 1.f;
   float scale2 = 1.f;
+  float scale3 = 0.f;
 };
 
 hipError_t launch_les(int which, const LesArgs& a, hipStream_t st);

@@ -46,11 +46,12 @@ def fft3(t: Any, L: int, nbatch: int, inverse: bool) -> None:
 
 
 def les_op(name: str, inp: Any, out: Any, *, n: int = 0, L: int = 0, L2: int = 0,
-           pitch: int = 0, scale: float = 1.0, scale2: float = 1.0, in2: Any = None) -> None:
+           pitch: int = 0, scale: float = 1.0, scale2: float = 1.0, scale3: float = 0.0,
+           in2: Any = None) -> None:
     """One of the LES support kernels (include/psx.h, PSX_LES_*)."""
     lib = _lib.load()
     a = _lib.LesArgs(inp=_p(inp).value, in2=_p(in2).value, out=_p(out).value, n=n, L=L, L2=L2,
-                     pitch=pitch, scale=scale, scale2=scale2)
+                     pitch=pitch, scale=scale, scale2=scale2, scale3=scale3)
     _lib.check(lib.psx_les_op(_lib.LES_OPS[name], ctypes.byref(a), _lib.stream_ptr()))
 
 
@@ -242,13 +243,15 @@ class LESSolver(HipSolver):
     and holds its divergence over the four stages.  ``run_device`` therefore drives an LES
     exactly as it drives the DNS.  ``backscatter`` multiplies the network's stress wherever it
     would transfer energy to the resolved scales (-tau_ij S_ij < 0): 1 keeps the network's
-    output as it is, 0 clips the backscatter."""
+    output as it is, 0 clips the backscatter, a factor up to 2 amplifies it.  ``mix_cs`` > 0 adds a Smagorinsky stress of that
+    constant to the network's (a mixed model); ``cs`` is the constant of the pure Smagorinsky
+    model."""
 
     variant = "les"
 
     def __init__(self, cfg: Config, model: str = "none", *, net: SgsNet | None = None,
                  cs: float = 0.17, kc: float = 21.0, g_scale: float = 1.0,
-                 tau_scale: float = 1.0, backscatter: float = 1.0) -> None:
+                 tau_scale: float = 1.0, backscatter: float = 1.0, mix_cs: float = 0.0) -> None:
         if cfg.dim != 3 or cfg.dtype != "float32":
             raise BackendUnavailable("LES: 3D fp32 flows")
         if model not in ("none", "smagorinsky", "net"):
@@ -260,6 +263,7 @@ class LESSolver(HipSolver):
         L = cfg.n
         self.model, self.net, self.cs, self.kc = model, net, cs, kc
         self.g_scale, self.tau_scale = g_scale, tau_scale
+        self.mix_cs = mix_cs
         self.src = torch.zeros_like(self.buf.u0_rows)
         self.full = torch.zeros(3 * L ** 3, dtype=torch.complex64, device="cuda")
         self.grad = torch.zeros(6 * L ** 3, dtype=torch.complex64, device="cuda")
@@ -271,7 +275,8 @@ class LESSolver(HipSolver):
                          full=_p(self.full).value, grad=_p(self.grad).value,
                          tau=_p(self.tau).value, x=_p(self.X).value,
                          g_scale=g_scale, tau_scale=tau_scale,
-                         smag=(cs * math.pi / kc) ** 2, backscatter=backscatter)
+                         smag=((cs if model == "smagorinsky" else mix_cs) * math.pi / kc) ** 2,
+                         backscatter=backscatter)
         if model == "net":
             assert net is not None and net.n == L
             acts = net._buffers(1)

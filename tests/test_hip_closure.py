@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 def test_trained_network_beats_no_model():
     print("case closure study (short)", flush=True)
     r = closure_study(t_spin=3.0, n_snap=40, snap_every=0.1, n_val=8, train_steps=200,
-                      t_les=6.0, t_avg=3.0, t_cost=1.0, backscatter=(1.0,),
+                      t_les=6.0, t_avg=3.0, t_cost=1.0, backscatter=(1.0,), mixed=((1.0, 0.04),),
                       log=lambda m: print("  " + m, flush=True))
     les = r["les"]
     assert all(les[m]["finite"] for m in les)

@@ -330,3 +330,64 @@ def test_network_stress_backscatter_clipping_matches_oracle():
     frac = float(np.mean(np.all(ref == 0, axis=0)))
     print(f"  clipped fraction {frac:.3f}", flush=True)
     assert 0.3 < frac < 0.7
+
+
+def test_mixed_model_stress_matches_oracle():
+    """PSX_LES_TAU_FROM_NET with scale3 = (Cs D)^2 > 0: the network's stress (after the
+    backscatter factor) plus the oracle's Smagorinsky stress of the same gradient."""
+    L = 64
+    L3 = L ** 3
+    rng = np.random.default_rng(37)
+    g = rng.normal(size=(3, 3, L3)).astype(np.float32)
+    vel = rng.normal(size=(3, L3)).astype(np.float32)
+    comps = [vel[0], vel[1], vel[2]] + [g[i, j] for i in range(3) for j in range(3)]
+    pairs = np.stack([comps[2 * p] + 1j * comps[2 * p + 1] for p in range(6)])
+    Y = rng.normal(size=(L3, C)).astype(np.float32)
+    yc = torch.from_numpy(Y).cuda()
+    pc = torch.from_numpy(pairs.astype(np.complex64).reshape(-1)).cuda()
+    t6 = (0.5 * Y[:, :6].T).astype(np.float64)
+    g64 = g.astype(np.float64)
+    csd2 = 0.02
+    smag = LO.smagorinsky(g64, csd2)
+    S = 0.5 * (g64 + np.swapaxes(g64, 0, 1))
+    ts = sum((1.0 if i == j else 2.0) * t6[c] * S[i, j]
+             for c, (i, j) in enumerate([(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]))
+    sure = np.abs(ts) > 1e-4 * np.abs(ts).max()
+    for keep in (1.0, 0.75):
+        want = LO.clip_backscatter(t6, g64, keep=keep) + smag
+        tau = torch.zeros(3 * L3, dtype=torch.complex64, device="cuda")
+        les_op("tau_from_net", pc, tau, L=L, scale=0.5, scale2=keep, scale3=csd2, in2=yc)
+        tp = tau.cpu().numpy().reshape(3, L3)
+        got = np.stack([tp[0].real, tp[0].imag, tp[1].real, tp[1].imag, tp[2].real, tp[2].imag])
+        err = np.abs(got - want)[:, sure].max() / np.abs(want).max()
+        print(f"  keep {keep}: max rel error {err:.2e}", flush=True)
+        assert err < 2e-6
+    # the Smagorinsky part is a real share of the stress in this test
+    assert np.abs(smag).max() > 0.3 * np.abs(t6).max()
+
+
+def test_mixed_model_les_step():
+    """An LES with the network plus a Smagorinsky part (mix_cs > 0) attached: its source
+    differs from the network-only source by the divergence of the Smagorinsky stress alone,
+    i.e. mixed - net equals the Smagorinsky model's source at the same constant."""
+    L = 64
+    cfg = Config(n=L, dim=3, nu=2e-3, dtype="float32")
+    st = _state3(L, 5)
+    net = SgsNet(seed=3)
+    srcs = {}
+    for key, model, kw in (("net", "net", {}), ("mixed", "net", {"mix_cs": 0.12}),
+                           ("smag", "smagorinsky", {"cs": 0.12})):
+        s = LESSolver(cfg, model, net=net if model == "net" else None, g_scale=0.5,
+                      tau_scale=1e-3, **kw)
+        s.set_state(st)
+        s.update_source()
+        torch.cuda.synchronize()
+        srcs[key] = s.src.cpu().numpy().astype(np.complex128)
+        s.close()
+    diff = srcs["mixed"] - srcs["net"]
+    big = max(np.abs(srcs["net"]).max(), np.abs(srcs["smag"]).max())
+    err = np.abs(diff - srcs["smag"]).max() / big
+    ratio = np.abs(srcs["smag"]).max() / big
+    print(f"  mixed - net vs Smagorinsky source: {err:.2e} (Smagorinsky share {ratio:.3f})",
+          flush=True)
+    assert ratio > 1e-2 and err < 1e-5
