@@ -149,6 +149,32 @@ PSX_HD void dft5(cx<T>* x) {
   x[3] = csub(a2, b2);
 }
 
+// 12 = 3 * 4 by the prime-factor (Good-Thomas) mapping, gcd(3, 4) = 1: 3-point DFTs over n1
+// for each n2 of input n = (4 n1 + 3 n2) mod 12, then 4-point DFTs over n2 for each k1 into
+// output k = (4 k1 + 9 k2) mod 12 (4 = 4 (4^-1 mod 3), 9 = 3 (3^-1 mod 4)); no twiddles
+// between the two stages.
+template <bool INV, typename T>
+PSX_HD void dft12(cx<T>* x) {
+  cx<T> a[3][4];
+  static_for<0, 4>([&](auto n2c) {
+    constexpr int n2 = decltype(n2c)::value;
+    cx<T> t[3] = {x[(3 * n2) % 12], x[(4 + 3 * n2) % 12], x[(8 + 3 * n2) % 12]};
+    dft3<INV>(t);
+    a[0][n2] = t[0];
+    a[1][n2] = t[1];
+    a[2][n2] = t[2];
+  });
+  static_for<0, 3>([&](auto k1c) {
+    constexpr int k1 = decltype(k1c)::value;
+    cx<T> t[4] = {a[k1][0], a[k1][1], a[k1][2], a[k1][3]};
+    dft4<INV>(t);
+    static_for<0, 4>([&](auto k2c) {
+      constexpr int k2 = decltype(k2c)::value;
+      x[(4 * k1 + 9 * k2) % 12] = t[k2];
+    });
+  });
+}
+
 // R = R1 * R2: inner R2-point DFTs over m1 (input m = m0 + R1 m1), twiddle w_R^{m0 q0},
 // outer R1-point DFTs over m0; output q = q0 + R2 q1.
 template <int R1, int R2, bool INV, typename T>
@@ -187,6 +213,8 @@ PSX_HD void dft(cx<T>* x) {
     dft5<INV>(x);
   } else if constexpr (R == 8) {
     dft_split<2, 4, INV>(x);
+  } else if constexpr (R == 12) {
+    dft12<INV>(x);
   } else if constexpr (R == 16) {
     dft_split<4, 4, INV>(x);
   } else if constexpr (R == 32) {

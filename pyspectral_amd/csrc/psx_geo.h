@@ -47,6 +47,18 @@ constexpr CtPlan ct_plan(int L, int E) {
   return p;
 }
 
+// As ct_plan, with one radix-12 pass (prime-factor 3 x 4 butterfly) in place of a radix-3
+// pass and two factors of 2, when 12 divides both L and E: one pass fewer for the 3 * 2^a
+// lengths (384 = 4 4 2 12 instead of 4 4 4 2 3).  Used by the compile-time 3D kernels.
+constexpr CtPlan ct_plan_r12(int L, int E) {
+  if (L % 12 || E % 12 || L == 12) return ct_plan(L, E);
+  CtPlan p = ct_plan(L / 12, E);
+  if (p.np == 0 || p.np >= kMaxRad) return ct_plan(L, E);
+  p.rad[p.np++] = 12;
+  p.L = L;
+  return p;
+}
+
 // Radix of pass q in the forward (inv = false) or reversed (inv = true) order.
 constexpr int ct_radix(const CtPlan& p, bool inv, int q) {
   return inv ? p.rad[p.np - 1 - q] : p.rad[q];
@@ -252,24 +264,25 @@ constexpr Geo3D geo3d(int n) {
     int t = 1;
     while (n * t / E < 128 && (n * n) % (t * 2) == 0) t *= 2;  // t divides the n^2 rows
     const int nt = n * t / E;
-    g.line = ct_plan(n, E);
+    g.line = ct_plan_r12(n, E);
     if (g.line.np == 0 || nt > 512 || (n * t) % E) return g;
     g.lineE = E;
     g.lineNT = nt;
     g.lineT = t;
   }
-  auto cols = [&](int E0, int& E, int& nt, CtPlan& pl) {
+  auto cols = [&](int E0, int& E, int& nt, CtPlan& pl, bool r12) {
     E = E0;
     while (E > 4 && ((n * T) % E || n * T / E > 512)) E = (E % 3 == 0 && E > 12) ? E / 2 : E;
-    pl = ct_plan(n, E);
+    pl = r12 ? ct_plan_r12(n, E) : ct_plan(n, E);
     nt = n * T / E;
     return pl.np > 0 && nt >= 1 && nt <= 512 && (n * T) % E == 0;
   };
   if (n * T * 8 > 65536) return g;
-  if (!cols(r3 ? 12 : (small ? 4 : 16), g.yE, g.yNT, g.yp)) return g;
+  if (!cols(r3 ? 12 : (small ? 4 : 16), g.yE, g.yNT, g.yp, true)) return g;
   // the x pass keeps three accumulated components, the state and the transform in registers:
-  // 8 elements per thread from 128 up (16 spilled past 256 VGPRs)
-  if (!cols(r3 ? 12 : (small ? 4 : (n >= 128 ? 8 : 16)), g.xE, g.xNT, g.xp)) return g;
+  // 8 elements per thread from 128 up (16 spilled past 256 VGPRs); radix-3 plans without the
+  // radix-12 pass, whose butterfly spills next to the accumulators at 384
+  if (!cols(r3 ? 12 : (small ? 4 : (n >= 128 ? 8 : 16)), g.xE, g.xNT, g.xp, false)) return g;
   g.ok = true;
   return g;
 }

@@ -29,7 +29,7 @@ def test_twiddles_are_rounded_fp64_values(L, inverse):
                               rad.ctypes.data_as(ctypes.c_void_p), ctypes.byref(npass))
     assert rc == 0, lib.psx_last_error()
     radices = [int(r) for r in rad[: npass.value]]
-    assert int(np.prod(radices)) == L and all(r in (2, 3, 4, 5, 8, 16, 32) for r in radices)
+    assert int(np.prod(radices)) == L and all(r in (2, 3, 4, 5, 8, 12, 16, 32) for r in radices)
     want = []
     ns = radices[0]
     for R in radices[1:]:
