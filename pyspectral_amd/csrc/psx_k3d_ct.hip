@@ -254,6 +254,9 @@ __global__ void __launch_bounds__(G3<N>::g.yNT, 1) kc_ypass3d(Flow<float> f) {
 }
 
 // ------------------------------------------------------------------ x pass with physics
+// Grid sizes whose x pass keeps its third accumulated component in LDS (see kc_spec3d).
+constexpr bool kc_spec3d_a2_lds(int n) { return n == 384; }
+
 template <int N, bool ROT, int stage>
 __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
   if (flow_inactive(f.prm, f.nh)) return;
@@ -278,6 +281,11 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
   constexpr int RS = ct_radix(G.xp, true, G.xp.np - 1);
   const BufC32 ub(f.u0, (unsigned long long)3 * SST * 8);
   const BufC32 ab(f.acc, (unsigned long long)3 * SST * 8);
+  // At 384 the third component lives in LDS slots of its own (thread-private, after the
+  // transform buffer): in registers beside the other two, the transform and the state it
+  // spilled to scratch.
+  constexpr bool kA2Lds = kc_spec3d_a2_lds(N);
+  cx<T>* a2l = lds + L * CT;
 
   // state offsets of this thread's modes (out of range outside the 2/3 mask)
   unsigned soff[E];
@@ -296,11 +304,18 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
     return soff[i] == 0xFFFFFFFFu ? kOob : (soff[i] + (unsigned)(c * SST)) * 8u;
   };
 
-  cx<T> a0[E], a1[E], a2[E];
+  cx<T> a0[E], a1[E], a2reg[kA2Lds ? 1 : E];
+  auto A2 = [&](int i) -> cx<T>& {
+    if constexpr (kA2Lds) {
+      return a2l[i * NT + tid];
+    } else {
+      return a2reg[i];
+    }
+  };
   if constexpr (stage != kStagePre) {
     constexpr int nf = ROT ? 3 : 5;
 #pragma unroll
-    for (int i = 0; i < E; ++i) a0[i] = a1[i] = a2[i] = czero<T>();
+    for (int i = 0; i < E; ++i) a0[i] = a1[i] = A2(i) = czero<T>();
     static_for<0, nf>([&](auto cc) {
       constexpr int c = decltype(cc)::value;
       cx<T> v[E];
@@ -316,7 +331,7 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
         if constexpr (ROT) {
           if constexpr (c == 0) a0[i] = v[i];
           else if constexpr (c == 1) a1[i] = v[i];
-          else a2[i] = v[i];
+          else A2(i) = v[i];
         } else {
           int e, t;
           ct_elem<RP, L, NT, Lay>(i, tid, e, t);
@@ -324,19 +339,19 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
           const cx<T> mt = mul_mi(v[i]);  // -i T_c;  N_i = -i k_j T_ij, T33 = -T11 - T22
           if constexpr (c == 0) {
             a0[i] = cadd(a0[i], cscl(kx, mt));
-            a2[i] = csub(a2[i], cscl(kz, mt));
+            A2(i) = csub(A2(i), cscl(kz, mt));
           } else if constexpr (c == 1) {
             a1[i] = cadd(a1[i], cscl(fy, mt));
-            a2[i] = csub(a2[i], cscl(kz, mt));
+            A2(i) = csub(A2(i), cscl(kz, mt));
           } else if constexpr (c == 2) {
             a0[i] = cadd(a0[i], cscl(fy, mt));
             a1[i] = cadd(a1[i], cscl(kx, mt));
           } else if constexpr (c == 3) {
             a0[i] = cadd(a0[i], cscl(kz, mt));
-            a2[i] = cadd(a2[i], cscl(kx, mt));
+            A2(i) = cadd(A2(i), cscl(kx, mt));
           } else {
             a1[i] = cadd(a1[i], cscl(kz, mt));
-            a2[i] = cadd(a2[i], cscl(fy, mt));
+            A2(i) = cadd(A2(i), cscl(fy, mt));
           }
         }
       }
@@ -371,12 +386,12 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
           const BufC32 sb(const_cast<cx<T>*>(f.src), (unsigned long long)3 * SST * 8);
           a0[i] = cadd(a0[i], sb.load(so(i, 0)));
           a1[i] = cadd(a1[i], sb.load(so(i, 1)));
-          a2[i] = cadd(a2[i], sb.load(so(i, 2)));
+          A2(i) = cadd(A2(i), sb.load(so(i, 2)));
         }
-        const cx<T> d = cscl(ik2, cadd(cadd(cscl(fx, a0[i]), cscl(fy, a1[i])), cscl(fz, a2[i])));
+        const cx<T> d = cscl(ik2, cadd(cadd(cscl(fx, a0[i]), cscl(fy, a1[i])), cscl(fz, A2(i))));
         const cx<T> n0 = csub(a0[i], cscl(fx, d));
         const cx<T> n1 = csub(a1[i], cscl(fy, d));
-        const cx<T> n2 = csub(a2[i], cscl(fz, d));
+        const cx<T> n2 = csub(A2(i), cscl(fz, d));
         double E1, Eh;
         if_factors(f.prm, NH, s, ax, ay, kz, k2, E1, Eh);
         cx<T> c0 = czero<T>(), c1 = czero<T>(), c2 = czero<T>();
@@ -404,7 +419,7 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
       }
       a0[i] = cscl(scale, ua);
       a1[i] = cscl(scale, ubv);
-      a2[i] = cscl(scale, uc);
+      A2(i) = cscl(scale, uc);
     }
   }
   constexpr int nin = ROT ? 6 : 3;
@@ -418,14 +433,14 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
       } else if constexpr (c == 1) {
         v[i] = a1[i];
       } else if constexpr (c == 2) {
-        v[i] = a2[i];
+        v[i] = A2(i);
       } else {
         int e, t;
         ct_elem<RP, L, NT, Lay>(i, tid, e, t);
         const T fx = (T)(e < N / 2 ? e : e - N), fy = (T)ky, fz = (T)(j0 + t);
         // omega = i k x u
-        if constexpr (c == 3) v[i] = mul_i(csub(cscl(fy, a2[i]), cscl(fz, a1[i])));
-        else if constexpr (c == 4) v[i] = mul_i(csub(cscl(fz, a0[i]), cscl(fx, a2[i])));
+        if constexpr (c == 3) v[i] = mul_i(csub(cscl(fy, A2(i)), cscl(fz, a1[i])));
+        else if constexpr (c == 4) v[i] = mul_i(csub(cscl(fz, a0[i]), cscl(fx, A2(i))));
         else v[i] = mul_i(csub(cscl(fx, a1[i]), cscl(fy, a0[i])));
       }
     }
@@ -476,8 +491,16 @@ static hipError_t ct3_launch(const Flow<float>& f, int which, int arg, hipStream
       break;
     }
     case 2: {
-      const size_t lds = (size_t)N * G.T * 8;
+      // transform buffer, plus the third component's slots where it lives in LDS
+      const size_t lds = (size_t)N * G.T * 8 +
+                         (kc_spec3d_a2_lds(N) ? (size_t)G.xE * G.xNT * 8 : 0);
       const dim3 grid(N * NJT), block(G.xNT);
+      static const bool attr_ = set_lds_max((const void*)kc_spec3d<N, ROT, kStagePre>) &&
+                                set_lds_max((const void*)kc_spec3d<N, ROT, 0>) &&
+                                set_lds_max((const void*)kc_spec3d<N, ROT, 1>) &&
+                                set_lds_max((const void*)kc_spec3d<N, ROT, 2>) &&
+                                set_lds_max((const void*)kc_spec3d<N, ROT, 3>);
+      (void)attr_;
       switch (arg) {
         case kStagePre: hipLaunchKernelGGL((kc_spec3d<N, ROT, kStagePre>), grid, block, lds, st, f); break;
         case 0: hipLaunchKernelGGL((kc_spec3d<N, ROT, 0>), grid, block, lds, st, f); break;

@@ -103,7 +103,9 @@ def test_2d_sizes_and_four_step(n, dtype, steps):
                                           (128, "basdevant", "float32"),
                                           (48, "rotational", "float64"),
                                           (32, "basdevant", "float64"),
-                                          (256, "basdevant", "float32")])
+                                          (256, "basdevant", "float32"),
+                                          (384, "basdevant", "float32"),
+                                          (384, "rotational", "float32")])
 def test_3d_sizes(n, form, dtype):
     print(f"case 3D n={n} {form} {dtype}", flush=True)
     cfg = Config(n=n, dim=3, nu=1 / 1600, dtype=dtype, form=form)
@@ -112,7 +114,7 @@ def test_3d_sizes(n, form, dtype):
     ic = initial.to_spectral(initial.taylor_green_3d(n), 3) + 0.01 * _ic(n, 3, k0=3, seed=4)
     s.set_state(ic)
     ref.set_state(ic)
-    steps = 1 if n > 128 else 3  # one FP64 oracle step at 256^3 takes seconds
+    steps = 1 if n > 128 else 3  # one FP64 oracle step at 256^3 takes seconds, 384^3 tens
     s.advance(steps, 0.02)
     ref.advance(steps, 0.02)
     err = O.rel_err(s.get_state(), ref.get_state())
