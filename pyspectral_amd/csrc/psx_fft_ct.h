@@ -15,10 +15,11 @@ namespace psx {
 // A single 8192-element row (passes of radix 32, 16, 16 and the reverse) has one pad element
 // per 32 instead: the radix-32 pass writes elements 32 apart, which one pad per 16 puts two
 // lanes of every 32 on the same bank pair (64 four-byte banks, 32 lanes of 8-byte accesses
-// at a time); measured 2 % faster on that kernel, where one pad per 8 is 3 % slower.
+// at a time); measured 2 % faster on that kernel, where one pad per 8 is 3 % slower.  The
+// 384-point rows of the 3D line kernel (radices 4 4 2 12) also use one per 32: 1.5 % faster.
 template <int L_, int T_>
 struct RowsCT {
-  static constexpr bool kPad32 = L_ == 8192 && T_ == 1;
+  static constexpr bool kPad32 = (L_ == 8192 && T_ == 1) || L_ == 384;
   static constexpr int L = L_, T = T_;
   static constexpr int LP = kPad32 ? L_ + (L_ >> 5) + 1 : ct_rows_pitch(L_);  // <= the latter
   static PSX_HD int addr(int e, int t) { return t * LP + e + (e >> (kPad32 ? 5 : 4)); }
