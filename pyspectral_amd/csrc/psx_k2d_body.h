@@ -145,11 +145,14 @@ __device__ __forceinline__ void body_spec2d(const Flow<float>& f, int tile, int 
   }
   // State and accumulator of this thread's modes, loaded before the transforms so their
   // latency hides behind them; modes outside the 2/3 mask read zero and are never written.
+  // With 16 elements per thread the accumulator is loaded after the transforms instead: held
+  // across them it pushed the kernel past 256 VGPRs into scratch.
+  constexpr bool kEarlyAcc = E <= 8;
   constexpr int RP = ct_radix(G.spec, false, G.spec.np - 1);
   const BufC32 ub(f.u0, (unsigned long long)N * SP * 8);
   const BufC32 ab(f.acc, (unsigned long long)N * SP * 8);
   unsigned soff[E];
-  cx<T> uu[E], aa[E];
+  cx<T> uu[E], aa[kEarlyAcc ? E : 1];
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     int e, t;
@@ -161,12 +164,14 @@ __device__ __forceinline__ void body_spec2d(const Flow<float>& f, int tile, int 
     soff[i] = (3 * ax < N && ky < JM) ? (unsigned)(k * SP + ky) * 8u : kOob;
     uu[i] = ub.load(soff[i]);
   }
-  if constexpr (stage >= 1) {
+  if constexpr (kEarlyAcc) {
+    if constexpr (stage >= 1) {
 #pragma unroll
-    for (int i = 0; i < E; ++i) aa[i] = ab.load(soff[i]);
-  } else {
+      for (int i = 0; i < E; ++i) aa[i] = ab.load(soff[i]);
+    } else {
 #pragma unroll
-    for (int i = 0; i < E; ++i) aa[i] = czero<T>();
+      for (int i = 0; i < E; ++i) aa[i] = czero<T>();
+    }
   }
   if constexpr (stage != kStagePre) {
     fft_ct<E, false, NT, Lay, G.spec>(va, lds, tw_fwd, tid);
@@ -191,7 +196,12 @@ __device__ __forceinline__ void body_spec2d(const Flow<float>& f, int tile, int 
         const cx<T> Nl = cadd(cscl(fx * fy, va[i]), cscl(fx * fx - fy * fy, vb[i]));
         double E1, Eh;
         if_factors(f.prm, NH, s, ax, ky, -1, k2, E1, Eh);
-        cx<T> acc = aa[i];
+        cx<T> acc = czero<T>();
+        if constexpr (kEarlyAcc) {
+          acc = aa[i];
+        } else if constexpr (stage >= 1) {
+          acc = ab.load(soff[i]);
+        }
         nxt = rk_value<stage, T>(dt, (T)E1, (T)Eh, Nl, uu[i], acc);
         if constexpr (stage < 3) ab.store(soff[i], acc);
         else ub.store(soff[i], nxt);
