@@ -58,11 +58,12 @@ def test_every_variant_matches_oracle():
 
 
 def test_member_equals_single_flow():
-    """256^2 with 16-column one-pass tiles uses the single-flow plans; the two kernels are
-    compiled separately (their fused multiply-adds may be formed differently), so the states
-    agree to fp32 round-off."""
+    """A 256^2 member (default variant: one pass, 8-column tiles, 8 elements per thread) and
+    the single-flow kernels (16-column tiles, 16 elements per thread) run different radix
+    plans, so the states agree to fp32 round-off of the transforms."""
     c = _cfg(256)
-    ens = EnsembleSolver([_cfg(64), c, _cfg(512)], [0, 0, variants_for(512)[0]])
+    ens = EnsembleSolver([_cfg(64), c, _cfg(512)],
+                         [0, variants_for(256)[0], variants_for(512)[0]])
     st = [_state(64, 1), _state(256, 2), _state(512, 3)]
     ens.set_states(st)
     s = get("hip/fused").make(c)
@@ -72,7 +73,7 @@ def test_member_equals_single_flow():
         s.step(0.004)
     err = O.rel_err(ens.get_state(1), s.get_state())
     print(f"  ensemble vs single flow rel err {err:.3e}", flush=True)
-    assert err < 1e-6
+    assert err < 2e-6
     ens.close()
 
 
