@@ -77,6 +77,7 @@ def _dns_and_snapshots(cfg_dns: Config, n_les: int, kc: float, nu: float, t_spin
     log(f"scales: velocity-gradient rms {g_rms:.4g}, stress rms {tau_rms:.4g}")
     les_probe = LESSolver(cfg_dns.with_(n=n_les), "none")
     ek_f = np.zeros(les_probe.diag.nshell)
+    ek_half = [np.zeros(les_probe.diag.nshell), np.zeros(les_probe.diag.nshell)]
     ek_dns = np.zeros(dns.diag.nshell)
     zf, e_dns, z_dns = [], [], []
     for i in range(n_snap):
@@ -88,12 +89,18 @@ def _dns_and_snapshots(cfg_dns: Config, n_les: int, kc: float, nu: float, t_spin
         ubar = pipe(dns.buf.u0_rows, dns.buf.pitch, X[i], T[i], g_scale=1.0 / g_rms,
                     tau_scale=1.0 / tau_rms)
         hs = pipe.half_state(ubar)
-        ek_f += _spectrum(les_probe, hs)
+        ek_s = _spectrum(les_probe, hs)
+        ek_f += ek_s
+        ek_half[0 if 2 * i < n_snap else 1] += ek_s
         zf.append(les_probe.diag.enstrophy(hs.contiguous()))
         if (i + 1) % 500 == 0:
             log(f"  snapshot {i + 1}/{n_snap} at t={dns.t:.2f}")
     ek_f /= n_snap
     ek_dns /= n_snap
+    # stationarity of the reference: the two halves of the sampling window
+    h0, h1 = ek_half[0] / max(1, (n_snap + 1) // 2), ek_half[1] / max(1, n_snap // 2)
+    kk = np.arange(1, int(kc) + 1)
+    half_diff = (h1[kk] - h0[kk]) / ek_f[kk]
     filt_state = pipe.half_state(pipe.ubar).cpu().numpy().astype(np.complex128)
     # the cost: one continuous device-controlled run, as the LES runs are timed
     torch.cuda.synchronize()
@@ -103,12 +110,15 @@ def _dns_and_snapshots(cfg_dns: Config, n_les: int, kc: float, nu: float, t_spin
     per_time = (time.perf_counter() - t0) / t_cost
     info = {"gpu_s_per_unit_time": per_time, "simulated_time": t_spin + n_snap * snap_every,
             "filtered_Ek": ek_f.tolist(), "filtered_2nuZ": 2 * nu * float(np.mean(zf)),
+            "filtered_Ek_halves": [h0.tolist(), h1.tolist()],
+            "filtered_Ek_half_diff_max": float(np.abs(half_diff).max()),
             "scales": {"g_rms": g_rms, "tau_rms": tau_rms},
             **turbulence_scales(float(np.mean(e_dns)), float(np.mean(z_dns)), ek_dns, nu,
                                 n_dns)}
     log(f"DNS: {n_snap} filtered snapshots; {per_time:.3f} GPU s per unit time; "
         f"Re_lambda {info['re_lambda']:.1f}, k_max eta {info['kmax_eta']:.2f}, "
-        f"large-eddy time {info['t_eddy']:.2f}")
+        f"large-eddy time {info['t_eddy']:.2f}; filtered E(k) second half vs first half of "
+        f"the window: max {info['filtered_Ek_half_diff_max']:.3f}")
     return {"info": info, "X": X, "T": T, "filt_state": filt_state}
 
 
