@@ -1,5 +1,5 @@
 """Learned-closure LES study: forced 3D DNS -> filtered snapshots -> network training -> coarse
-LES with no model, Smagorinsky and the network, judged against the filtered DNS.
+LES with no model, Smagorinsky and the network, compared with the filtered DNS.
 
     python -m pyspectral_amd.closure [--quick] [--out results.json]
 

@@ -136,8 +136,8 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("mode", choices=("gpu", "oracle"))
     ap.add_argument("--part", type=int, default=0, choices=range(PARTS))
-    ap.add_argument("--out", default="gpurun_out/parity8192")
-    ap.add_argument("--inp", default="gpurun_out/parity8192")
+    ap.add_argument("--out", default="parity8192_out", help="where the GPU parts are written")
+    ap.add_argument("--inp", default="parity8192_out", help="where the oracle reads them")
     ap.add_argument("--steps", type=int, default=STEPS)
     ap.add_argument("--workers", type=int, default=os.cpu_count() or 1)
     a = ap.parse_args()
