@@ -1,6 +1,6 @@
 """Shared test configuration: the ``gpu`` marker and the repo root on ``sys.path``.
 
-GPU tests are skipped when torch sees no HIP device (the CPU container); on a GPU box they
+GPU tests are skipped when torch sees no HIP device (a machine without a GPU); with a GPU they
 run the HIP path and fail loudly if ``libpsx.so`` is missing.
 """
 

@@ -26,8 +26,8 @@ VARIANTS = ["hip/fused", "hip/fused-graph"]
 def _make(name, cfg):
     try:
         return get(name).make(cfg)
-    except BackendUnavailable as e:  # a GPU box without the library must fail, not skip
-        raise AssertionError(f"{name} unavailable on a GPU box: {e}") from e
+    except BackendUnavailable as e:  # a GPU machine without the library must fail, not skip
+        raise AssertionError(f"{name} unavailable with a GPU present: {e}") from e
 
 
 def _ic(n, dim, k0=None, seed=0):

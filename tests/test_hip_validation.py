@@ -7,7 +7,7 @@
 - IF-RK4 temporal self-convergence on a smooth nonlinear flow (validation.py:86-100): order
   near 4 and the published error at every dt (results/validation_2d.json, "temporal").
 - Spectral convergence in n (validation.py:103-119) against the published errors at the grid
-  sizes this build supports (2^a and 3 * 2^a; results/validation_2d.json, "spatial").
+  sizes the kernels support (2^a and 3 * 2^a; results/validation_2d.json, "spatial").
 
 Maximum relative errors are max|a - b| / max|b| (validation.py:36-38).
 """
@@ -40,8 +40,8 @@ PUB_SPATIAL = {8: 0.09457572679027483, 12: 0.06184973471137069, 16: 0.0054218778
 def _make(cfg, name="hip/fused"):
     try:
         return get(name).make(cfg)
-    except BackendUnavailable as e:  # a GPU box without the library must fail, not skip
-        raise AssertionError(f"{name} unavailable on a GPU box: {e}") from e
+    except BackendUnavailable as e:  # a GPU machine without the library must fail, not skip
+        raise AssertionError(f"{name} unavailable with a GPU present: {e}") from e
 
 
 def _slope(x, y):
