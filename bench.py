@@ -370,7 +370,7 @@ def main(argv: list[str] | None = None) -> int:
     a = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: this build advances a single flow on one "
+        print("bench.py measures one GPU only: the path advances a single flow on one "
               "MI355X (multi-GPU sharding is a design note, not a run); use --gpus 1",
               file=sys.stderr)
         return 2
