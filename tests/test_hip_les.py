@@ -391,3 +391,18 @@ def test_mixed_model_les_step():
     print(f"  mixed - net vs Smagorinsky source: {err:.2e} (Smagorinsky share {ratio:.3f})",
           flush=True)
     assert ratio > 1e-2 and err < 1e-5
+
+
+def test_sgs_attach_rejects_bad_coefficients():
+    """psx_set_sgs refuses a backscatter factor outside [0, 2] and a negative Smagorinsky
+    coefficient (the mixed model's constant), with the library's error message."""
+    cfg = Config(n=64, dim=3, nu=2e-3, dtype="float32")
+    net = SgsNet(seed=5)
+    for kw, msg in (({"backscatter": 2.5}, "backscatter factor"),
+                    ({"backscatter": -0.1}, "backscatter factor"),
+                    ({"mix_cs": float("nan")}, "Smagorinsky")):
+        with pytest.raises(RuntimeError, match=msg):
+            LESSolver(cfg, "net", net=net, g_scale=0.5, tau_scale=1e-3, **kw)
+    ok = LESSolver(cfg, "net", net=net, g_scale=0.5, tau_scale=1e-3, backscatter=2.0,
+                   mix_cs=0.04)
+    ok.close()
