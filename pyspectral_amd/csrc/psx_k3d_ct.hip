@@ -179,6 +179,15 @@ __global__ void __launch_bounds__(G3<N>::g.lineNT, 1) kc_line3d(Flow<float> f, i
 // --------------------------------------------------------------------------- y passes
 // Tile: fixed x (group), columns j0..j0+T, rows x N + y.  Forward: split packed pairs into the
 // two real fields' spectra, transform along y.  Inverse: transform along y, pack pairs.
+// Non-temporal 8-byte load of a work-field element: the y and x passes read each element once
+// (384^3: forward y pass 405 -> 356 us, x pass 748 -> 724 us, step 8.65 -> 8.43 ms; the stores
+// stay ordinary, as the next kernel reads them).
+typedef float psx_f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ cx<float> load_nt(const cx<float>* p) {
+  const psx_f2 v = __builtin_nontemporal_load(reinterpret_cast<const psx_f2*>(p));
+  return mkc(v.x, v.y);
+}
+
 template <int N, bool ROT, bool INV>
 __global__ void __launch_bounds__(G3<N>::g.yNT, 1) kc_ypass3d(Flow<float> f) {
   if (flow_inactive(f.prm, f.nh)) return;
@@ -208,8 +217,8 @@ __global__ void __launch_bounds__(G3<N>::g.yNT, 1) kc_ypass3d(Flow<float> f) {
       int e, t;
       ct_elem<R0, L, NT, Lay>(i, tid, e, t);
       const long long a = (rb + e) * LDW + j0 + t;
-      const cx<T> z = f.W[fa * FS + a];
-      const cx<T> zm = f.W[fb * FS + a];
+      const cx<T> z = load_nt(f.W + fa * FS + a);
+      const cx<T> zm = load_nt(f.W + fb * FS + a);
       if constexpr (INV) {
         va[i] = z;
         vb[i] = zm;
@@ -241,7 +250,7 @@ __global__ void __launch_bounds__(G3<N>::g.yNT, 1) kc_ypass3d(Flow<float> f) {
     for (int i = 0; i < E; ++i) {
       int e, t;
       ct_elem<R0, L, NT, Lay>(i, tid, e, t);
-      v[i] = f.W[SINGLE * FS + (rb + e) * LDW + j0 + t];
+      v[i] = load_nt(f.W + SINGLE * FS + (rb + e) * LDW + j0 + t);
     }
     fft_ct<E, INV, NT, Lay, G.yp>(v, lds, tw_o, tid);
 #pragma unroll
@@ -323,7 +332,7 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
       for (int i = 0; i < E; ++i) {
         int e, t;
         ct_elem<R0, L, NT, Lay>(i, tid, e, t);
-        v[i] = f.W[c * FS + ((long long)e * N + iy) * LDW + j0 + t];
+        v[i] = load_nt(f.W + c * FS + ((long long)e * N + iy) * LDW + j0 + t);
       }
       fft_ct<E, false, NT, Lay, G.xp>(v, lds, tw_fwd, tid);
 #pragma unroll
