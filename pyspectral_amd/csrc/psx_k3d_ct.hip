@@ -15,6 +15,11 @@
 
 namespace psx {
 
+// Cache policy of the 3D kernels' single-use accesses: non-temporal from 384^3 up; smaller
+// grids' fields fit the caches (the 64^3 LES step and the 256^3 DNS were slower with it).
+constexpr int kc3_pol(int n) { return n >= 384 ? kNt : kCached; }
+
+
 template <int N>
 struct G3 {
   static constexpr Geo3D g = geo3d(N);
@@ -58,7 +63,7 @@ __global__ void __launch_bounds__(G3<N>::g.lineNT, 1) kc_line3d(Flow<float> f, i
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int tid = threadIdx.x;
   const long long row0 = (long long)blockIdx.x * RT;
-  const BufC32 wb(f.W, (unsigned long long)NF * FS * 8);
+  const BufC32T<kc3_pol(N)> wb(f.W, (unsigned long long)NF * FS * 8);
   const cx<T>* tw_fwd = f.twp + f.prow.twoff[0][0];
   const cx<T>* tw_rev = f.twp + f.prow.twoff[1][0];
   constexpr int RI = ct_radix(G.line, true, 0);
@@ -176,18 +181,23 @@ __global__ void __launch_bounds__(G3<N>::g.lineNT, 1) kc_line3d(Flow<float> f, i
   }
 }
 
+// 8-byte load of a work-field element, non-temporal where the grid's policy says so: the y and
+// x passes read each element once (384^3: forward y pass 405 -> 356 us, x pass 748 -> 724 us,
+// step 8.65 -> 8.43 ms; their stores stay ordinary, as the next kernel reads them).
+typedef float psx_f2 __attribute__((ext_vector_type(2)));
+template <int N>
+__device__ __forceinline__ cx<float> load_w(const cx<float>* p) {
+  if constexpr (kc3_pol(N) == kNt) {
+    const psx_f2 v = __builtin_nontemporal_load(reinterpret_cast<const psx_f2*>(p));
+    return mkc(v.x, v.y);
+  } else {
+    return *p;
+  }
+}
+
 // --------------------------------------------------------------------------- y passes
 // Tile: fixed x (group), columns j0..j0+T, rows x N + y.  Forward: split packed pairs into the
 // two real fields' spectra, transform along y.  Inverse: transform along y, pack pairs.
-// Non-temporal 8-byte load of a work-field element: the y and x passes read each element once
-// (384^3: forward y pass 405 -> 356 us, x pass 748 -> 724 us, step 8.65 -> 8.43 ms; the stores
-// stay ordinary, as the next kernel reads them).
-typedef float psx_f2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ cx<float> load_nt(const cx<float>* p) {
-  const psx_f2 v = __builtin_nontemporal_load(reinterpret_cast<const psx_f2*>(p));
-  return mkc(v.x, v.y);
-}
-
 template <int N, bool ROT, bool INV>
 __global__ void __launch_bounds__(G3<N>::g.yNT, 1) kc_ypass3d(Flow<float> f) {
   if (flow_inactive(f.prm, f.nh)) return;
@@ -217,8 +227,8 @@ __global__ void __launch_bounds__(G3<N>::g.yNT, 1) kc_ypass3d(Flow<float> f) {
       int e, t;
       ct_elem<R0, L, NT, Lay>(i, tid, e, t);
       const long long a = (rb + e) * LDW + j0 + t;
-      const cx<T> z = load_nt(f.W + fa * FS + a);
-      const cx<T> zm = load_nt(f.W + fb * FS + a);
+      const cx<T> z = load_w<N>(f.W + fa * FS + a);
+      const cx<T> zm = load_w<N>(f.W + fb * FS + a);
       if constexpr (INV) {
         va[i] = z;
         vb[i] = zm;
@@ -250,7 +260,7 @@ __global__ void __launch_bounds__(G3<N>::g.yNT, 1) kc_ypass3d(Flow<float> f) {
     for (int i = 0; i < E; ++i) {
       int e, t;
       ct_elem<R0, L, NT, Lay>(i, tid, e, t);
-      v[i] = load_nt(f.W + SINGLE * FS + (rb + e) * LDW + j0 + t);
+      v[i] = load_w<N>(f.W + SINGLE * FS + (rb + e) * LDW + j0 + t);
     }
     fft_ct<E, INV, NT, Lay, G.yp>(v, lds, tw_o, tid);
 #pragma unroll
@@ -288,8 +298,8 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
   constexpr int R0 = ct_radix(G.xp, false, 0);
   constexpr int RP = ct_radix(G.xp, false, G.xp.np - 1);
   constexpr int RS = ct_radix(G.xp, true, G.xp.np - 1);
-  const BufC32 ub(f.u0, (unsigned long long)3 * SST * 8);
-  const BufC32 ab(f.acc, (unsigned long long)3 * SST * 8);
+  const BufC32T<kc3_pol(N)> ub(f.u0, (unsigned long long)3 * SST * 8);
+  const BufC32T<kc3_pol(N)> ab(f.acc, (unsigned long long)3 * SST * 8);
   // At 384 the third component lives in LDS slots of its own (thread-private, after the
   // transform buffer): in registers beside the other two, the transform and the state it
   // spilled to scratch.
@@ -332,7 +342,7 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
       for (int i = 0; i < E; ++i) {
         int e, t;
         ct_elem<R0, L, NT, Lay>(i, tid, e, t);
-        v[i] = load_nt(f.W + c * FS + ((long long)e * N + iy) * LDW + j0 + t);
+        v[i] = load_w<N>(f.W + c * FS + ((long long)e * N + iy) * LDW + j0 + t);
       }
       fft_ct<E, false, NT, Lay, G.xp>(v, lds, tw_fwd, tid);
 #pragma unroll
@@ -392,7 +402,7 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
         const T fx = (T)kx, fy = (T)ky, fz = (T)kz;
         const T ik2 = k2 > 0 ? __builtin_amdgcn_rcpf((T)k2) : (T)0;
         if (f.src) {  // uniform: an additive source (state layout, zero outside the mask)
-          const BufC32 sb(const_cast<cx<T>*>(f.src), (unsigned long long)3 * SST * 8);
+          const BufC32T<kc3_pol(N)> sb(const_cast<cx<T>*>(f.src), (unsigned long long)3 * SST * 8);
           a0[i] = cadd(a0[i], sb.load(so(i, 0)));
           a1[i] = cadd(a1[i], sb.load(so(i, 1)));
           A2(i) = cadd(A2(i), sb.load(so(i, 2)));

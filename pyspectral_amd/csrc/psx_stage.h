@@ -106,29 +106,34 @@ __device__ __forceinline__ cx<T> rk_value(T dt, T E, T Eh, cx<T> N, cx<T> u, cx<
 // compiler merges into one 64-bit access (the 64-bit raw buffer load builtin of this toolchain
 // loads only the first dword).
 constexpr unsigned kOob = 0x80000000u;
-// Cache policy of these accesses: non-temporal (the NT bit of the buffer instruction).  The
-// state, the accumulator and the packed line spectra are each read and written once per stage,
-// so keeping them out of the caches leaves room for the data that is reused; measured 1.5 % on
-// the 8192^2 step, 2.6 % on 384^3, 6.8 % on the forced ensemble, results bit-identical.
+// Cache policy of these accesses (template argument POL, the buffer instruction's cache-policy
+// bits): non-temporal (kNt) by default.  The state, the accumulator and the packed line
+// spectra are each read and written once per stage, so keeping them out of the caches leaves
+// room for the data that is reused: measured 1.5 % on the 8192^2 step, 2.6 % on 384^3 and
+// 6.8 % on the forced ensemble, results bit-identical.  Grids whose fields fit the caches
+// (3D below 384^3: the 64^3 LES, the 256^3 DNS) were slower with it and use kCached.
 constexpr int kNt = 2;
+constexpr int kCached = 0;
 
-struct BufC32 {
+template <int POL = kNt>
+struct BufC32T {
   __amdgpu_buffer_rsrc_t r;
-  __device__ __forceinline__ BufC32(const void* base, unsigned long long bytes)
+  __device__ __forceinline__ BufC32T(const void* base, unsigned long long bytes)
       : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0,
                                             (int)(bytes < kOob ? bytes : kOob - 8), 0x00020000)) {}
   __device__ __forceinline__ cx<float> load(unsigned off) const {
-    const unsigned a = __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, kNt);
-    const unsigned b = __builtin_amdgcn_raw_buffer_load_b32(r, (int)(off + 4u), 0, kNt);
+    const unsigned a = __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, POL);
+    const unsigned b = __builtin_amdgcn_raw_buffer_load_b32(r, (int)(off + 4u), 0, POL);
     return mkc(__builtin_bit_cast(float, a), __builtin_bit_cast(float, b));
   }
   __device__ __forceinline__ void store(unsigned off, cx<float> x) const {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x.x), r, (int)off, 0,
-                                          kNt);
+                                          POL);
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x.y), r, (int)(off + 4u),
-                                          0, kNt);
+                                          0, POL);
   }
 };
+using BufC32 = BufC32T<kNt>;
 
 // Block-wide max of NV non-negative values, folded into stats[] with one atomic per
 // work-group (bit patterns of non-negative doubles order like unsigned 64-bit integers).
