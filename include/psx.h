@@ -146,6 +146,9 @@ typedef struct psx_run_params {
   int cap_step;     /* capacity of the step ring (h, A) */
   int record_start; /* 1: record the initial state */
   double next_diag; /* first record time when diag_dt > 0 (NaN: t0 + diag_dt) */
+  double last_rec_t; /* time of the last record of an earlier part of the same history (NaN:
+                        none); a run that begins at t_end records its end state only when that
+                        record is older than t0, as simulate.py:93-95 does */
 } psx_run_params;
 
 typedef struct psx_run_state {

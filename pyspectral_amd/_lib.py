@@ -32,7 +32,7 @@ class RunParams(ctypes.Structure):
                 ("dt_max", c_double), ("diag_dt", c_double), ("power", c_double),
                 ("amp0", c_double), ("kmax", c_longlong), ("cfl_every", c_int),
                 ("cap_rec", c_int), ("cap_step", c_int), ("record_start", c_int),
-                ("next_diag", c_double)]
+                ("next_diag", c_double), ("last_rec_t", c_double)]
 
 
 class RunState(ctypes.Structure):

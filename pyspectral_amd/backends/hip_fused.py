@@ -29,6 +29,17 @@ def flow_params(cfg: Config, form: str) -> _lib.FlowParams:
                            klo2=cfg.force_klo**2, khi2=cfg.force_khi**2)
 
 
+def check_ring_bounds(chunk: int, depth: int, cap_step: int, cap_rec: int) -> None:
+    """The device loop's rings are drained before half of either can be overwritten: every
+    step writes one step entry and at most one record, and up to ``depth + 1`` chunks run
+    between two drains (plus the initial record)."""
+    if chunk < 1 or depth < 1:
+        raise ValueError("chunk and depth must be >= 1")
+    if chunk * (depth + 1) + 1 > min(cap_step, cap_rec) // 2:
+        raise ValueError(f"chunk * (depth + 1) = {chunk * (depth + 1)} exceeds half the "
+                         f"step ring ({cap_step}) or the record ring ({cap_rec})")
+
+
 class _Buffers:
     """Device buffers plus the C context that borrows them."""
 
@@ -164,17 +175,19 @@ class HipSolver(SpectralSolver):
                    dt_max: float = 0.05, cfl_every: int = 10, diag_dt: float | None = None,
                    target_power: float | None = None, kmax: int = 0, chunk: int | None = None,
                    use_graph: bool = True, depth: int = 3, record_start: bool = True,
-                   first_diag: float | None = None) -> dict[str, Any]:
+                   first_diag: float | None = None,
+                   last_record: float | None = None) -> dict[str, Any]:
         """Integrate to ``t_end`` (or ``kmax`` steps) with the controller on the device
         (src/pyspectral/simulate.py:40-96 semantics, no callback).  Chunks of ``chunk`` steps
         are enqueued ``depth`` deep; the host waits only on the oldest chunk's state copy.
+        ``last_record``: time of the last record of an earlier part of the same history (a
+        continued run records its end state only when that one is older).
         Returns the records (t, E, Z) and the step history (h, A)."""
         lib, st = self.lib, _lib.stream_ptr()
         chunk = int(chunk or cfl_every)
-        cap_step = 1 << 14
-        cap_rec = 1 << 12
-        if chunk * (depth + 1) > cap_step // 2:
-            raise ValueError("chunk * (depth + 1) exceeds the step ring")
+        # a step writes one step entry and at most one record: both rings take the same bound
+        cap_step = cap_rec = 1 << 14
+        check_ring_bounds(chunk, depth, cap_step, cap_rec)
         rp = _lib.RunParams(t0=float(self.t), t_end=float(t_end),
                             dt_fixed=float(dt) if dt is not None else 0.0, cfl=float(cfl),
                             dt_max=float(dt_max), diag_dt=float(diag_dt or 0.0),
@@ -182,7 +195,9 @@ class HipSolver(SpectralSolver):
                             amp0=float(self.force_amp), kmax=int(kmax),
                             cfl_every=int(cfl_every), cap_rec=cap_rec, cap_step=cap_step,
                             record_start=int(record_start),
-                            next_diag=float(first_diag) if first_diag is not None else math.nan)
+                            next_diag=float(first_diag) if first_diag is not None else math.nan,
+                            last_rec_t=float(last_record) if last_record is not None
+                            else math.nan)
         _lib.check(lib.psx_run_begin(self.ctx, ctypes.byref(rp), st))
         self._key = None  # the controller rewrites the parameter block
         rec = np.zeros(3 * cap_rec)

@@ -802,20 +802,33 @@ int psx_run_begin(psx_ctx* c, const psx_run_params* rp, void* stream) {
   const DrvState s = drv_initial_state(*rp, c->p.n);
   c->hmem = m;
   c->run_mode = s.dt_fixed > 0.0 ? 1 : 2;
-  PSX_HIP(hipMemcpyAsync(c->dmem, &c->hmem, sizeof(DrvMember), hipMemcpyHostToDevice, st));
-  PSX_HIP(hipMemcpyAsync(c->dstate, &s, sizeof(DrvState), hipMemcpyHostToDevice, st));
-  PSX_HIP(launch_fill_u32(reinterpret_cast<unsigned*>(c->stats), 0u, 6, st));
-  if (s.active) {
-    // the parameter block marks the flow active; the controller rewrites it every step
-    const int nh = c->p.n / 2 + 1;
-    PSX_HIP(launch_params(c->prm, nh, c->p.nu, 0.0, c->p.alpha, rp->amp0, c->p.klo2,
-                          c->p.khi2, c->p.ka2, st));
-  } else {
-    PSX_HIP(launch_drv(4, c->dmem, 1, st));
+  // The copies read `s` in this frame asynchronously: the stream is drained before
+  // returning, on the error paths too.
+  auto enqueue = [&]() -> int {
+    PSX_HIP(hipMemcpyAsync(c->dmem, &c->hmem, sizeof(DrvMember), hipMemcpyHostToDevice, st));
+    PSX_HIP(hipMemcpyAsync(c->dstate, &s, sizeof(DrvState), hipMemcpyHostToDevice, st));
+    PSX_HIP(launch_fill_u32(reinterpret_cast<unsigned*>(c->stats), 0u, 6, st));
+    if (s.active) {
+      // the parameter block marks the flow active; the controller rewrites it every step
+      const int nh = c->p.n / 2 + 1;
+      PSX_HIP(launch_params(c->prm, nh, c->p.nu, 0.0, c->p.alpha, rp->amp0, c->p.klo2,
+                            c->p.khi2, c->p.ka2, st));
+    } else {
+      PSX_HIP(launch_drv(4, c->dmem, 1, st));
+    }
+    PSX_HIP(launch_drv(2, c->dmem, 1, st));  // initial (or end) record when due
+    return 0;
+  };
+  const int r = enqueue();
+  const hipError_t es = hipStreamSynchronize(st);
+  if (r) {
+    c->run_mode = 0;
+    return r;
   }
-  PSX_HIP(launch_drv(2, c->dmem, 1, st));  // initial record when asked
-  // the host copies may be read by the kernels only after they land
-  PSX_HIP(hipStreamSynchronize(st));
+  if (es != hipSuccess) {
+    c->run_mode = 0;
+    return fail("psx_run_begin: %s", hipGetErrorString(es));
+  }
   return 0;
 }
 

@@ -490,28 +490,44 @@ int psx_ens_run_begin(psx_ens* e, const psx_run_params* rp, void* stream) {
     hs[m] = drv_initial_state(rp[m], p.n);
   }
   e->run_mode = r0.dt_fixed > 0.0 ? 1 : 2;
-  PSX_HIP(hipMemcpyAsync(e->dmem, e->hmem.data(), c * sizeof(DrvMember), hipMemcpyHostToDevice, st));
-  PSX_HIP(hipMemcpyAsync(e->dstate, hs.data(), c * sizeof(DrvState), hipMemcpyHostToDevice, st));
-  PSX_HIP(launch_fill_u32(reinterpret_cast<unsigned*>(e->stats), 0u, 8 * c, st));
-  for (int m = 0; m < c; ++m) {
-    const psx_flow_params& p = e->p[m];
-    PSX_HIP(launch_params(e->prm + e->prm_off[m], p.n / 2 + 1, p.nu, 0.0, p.alpha, rp[m].amp0,
-                          p.klo2, p.khi2, p.ka2, st));
-  }
   // members over before their first step are marked inactive by the set-active kernel
   std::vector<DrvMember> idle;
   for (int m = 0; m < c; ++m)
     if (!hs[m].active) idle.push_back(e->hmem[m]);
   DrvMember* didle = nullptr;
-  if (!idle.empty()) {
-    PSX_HIP(hipMalloc(&didle, idle.size() * sizeof(DrvMember)));
-    PSX_HIP(hipMemcpyAsync(didle, idle.data(), idle.size() * sizeof(DrvMember),
+  if (!idle.empty()) PSX_HIP(hipMalloc(&didle, idle.size() * sizeof(DrvMember)));
+  // The copies read hs and idle in this frame asynchronously: the stream is drained before
+  // returning, on the error paths too.
+  auto enqueue = [&]() -> int {
+    PSX_HIP(hipMemcpyAsync(e->dmem, e->hmem.data(), c * sizeof(DrvMember),
                            hipMemcpyHostToDevice, st));
-    PSX_HIP(launch_drv(4, didle, (int)idle.size(), st));
-  }
-  PSX_HIP(launch_drv(2, e->dmem, c, st));
-  PSX_HIP(hipStreamSynchronize(st));
+    PSX_HIP(hipMemcpyAsync(e->dstate, hs.data(), c * sizeof(DrvState), hipMemcpyHostToDevice,
+                           st));
+    PSX_HIP(launch_fill_u32(reinterpret_cast<unsigned*>(e->stats), 0u, 8 * c, st));
+    for (int m = 0; m < c; ++m) {
+      const psx_flow_params& p = e->p[m];
+      PSX_HIP(launch_params(e->prm + e->prm_off[m], p.n / 2 + 1, p.nu, 0.0, p.alpha,
+                            rp[m].amp0, p.klo2, p.khi2, p.ka2, st));
+    }
+    if (didle) {
+      PSX_HIP(hipMemcpyAsync(didle, idle.data(), idle.size() * sizeof(DrvMember),
+                             hipMemcpyHostToDevice, st));
+      PSX_HIP(launch_drv(4, didle, (int)idle.size(), st));
+    }
+    PSX_HIP(launch_drv(2, e->dmem, c, st));
+    return 0;
+  };
+  const int r = enqueue();
+  const hipError_t es = hipStreamSynchronize(st);
   if (didle) (void)hipFree(didle);
+  if (r) {
+    e->run_mode = 0;
+    return r;
+  }
+  if (es != hipSuccess) {
+    e->run_mode = 0;
+    return fail("psx_ens_run_begin: %s", hipGetErrorString(es));
+  }
   return 0;
 }
 

@@ -103,14 +103,16 @@ inline DrvState drv_initial_state(const psx_run_params& rp, int n) {
   s.amp = rp.amp0;
   s.cur_dt = s.dt_fixed;
   s.h = 0.0;
-  s.last_rec_t = -INFINITY;
+  s.last_rec_t = std::isfinite(rp.last_rec_t) ? rp.last_rec_t : -INFINITY;
   s.k = 0;
   s.kmax = rp.kmax > 0 ? rp.kmax : (long long)9e18;
   s.cfl_every = rp.cfl_every;
   s.nrec = 0;
   s.done = !(s.t < s.t_end - s.eps);
   s.active = !s.done;
-  s.record = rp.record_start ? 1 : 0;
+  // a run over before its first step still records its end state when the last record of
+  // the history is older than t (the end-of-run rule of the step kernels)
+  s.record = (rp.record_start || (s.done && s.last_rec_t < s.t - s.eps)) ? 1 : 0;
   return s;
 }
 

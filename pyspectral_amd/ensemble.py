@@ -26,7 +26,7 @@ import numpy as np
 
 from pyspectral_amd import _lib
 from pyspectral_amd.backends.base import BackendUnavailable
-from pyspectral_amd.backends.hip_fused import flow_params
+from pyspectral_amd.backends.hip_fused import check_ring_bounds, flow_params
 from pyspectral_amd.config import Config
 from pyspectral_amd.grid import dealias_mask
 
@@ -147,9 +147,9 @@ class EnsembleSolver:
         and the step history (h, A)."""
         lib, st, c = self.lib, _lib.stream_ptr(), self.count
         chunk = int(chunk or cfl_every)
-        cap_step, cap_rec = 1 << 12, 1 << 10
-        if chunk * (depth + 1) > cap_step // 2:
-            raise ValueError("chunk * (depth + 1) exceeds the step ring")
+        # a step writes one step entry and at most one record: both rings take the same bound
+        cap_step = cap_rec = 1 << 12
+        check_ring_bounds(chunk, depth, cap_step, cap_rec)
         ends = [float(t_end)] * c if np.isscalar(t_end) else [float(x) for x in t_end]
         rp = (_lib.RunParams * c)()
         for m in range(c):
@@ -159,7 +159,7 @@ class EnsembleSolver:
                 power=-1.0 if target_power is None else float(target_power),
                 amp0=self.force_amp[m], kmax=int(kmax), cfl_every=int(cfl_every),
                 cap_rec=cap_rec, cap_step=cap_step, record_start=int(record_start),
-                next_diag=math.nan)
+                next_diag=math.nan, last_rec_t=math.nan)
         _lib.check(lib.psx_ens_run_begin(self.ctx, rp, st))
         self._key = None
         rec = np.zeros(c * 3 * cap_rec)

@@ -92,7 +92,7 @@ def _run_on_device(solver: Any, t_end: float, res: RunResult, **kw: Any) -> RunR
     # a continued run keeps the record schedule of its earlier part
     first = res.t[-1] + diag_dt if (res.t and diag_dt) else None
     out = solver.run_device(t_end, diag_dt=diag_dt, first_diag=first, record_start=not res.t,
-                            **kw)
+                            last_record=res.t[-1] if res.t else None, **kw)
     res.t += out["t"]
     res.energy += out["energy"]
     res.enstrophy += out["enstrophy"]

@@ -175,3 +175,18 @@ def test_checkpoint_round_trip(tmp_path):
     assert np.array_equal(b.get_state(), a.get_state()) and b.t == a.t
     assert b.force_amp == 0.35 and r2.as_dict() == res.as_dict()
     assert np.array_equal(z2["spectrum"], np.arange(4.0))
+
+
+def test_device_loop_ring_bounds():
+    """The device loop refuses chunk sizes that could overwrite undrained ring entries."""
+    import pytest
+
+    from pyspectral_amd.backends.hip_fused import check_ring_bounds
+
+    check_ring_bounds(1500, 3, 1 << 14, 1 << 14)
+    with pytest.raises(ValueError):
+        check_ring_bounds(2048, 3, 1 << 14, 1 << 14)
+    with pytest.raises(ValueError):
+        check_ring_bounds(342, 3, 1 << 12, 1 << 10)
+    with pytest.raises(ValueError):
+        check_ring_bounds(0, 3, 1 << 14, 1 << 14)

@@ -122,18 +122,77 @@ def test_step_budget_and_eager_steps_after_a_run():
 
 
 def test_full_size_controller_run_8192():
-    """The 8192^2 fp32 workload under the device loop: CFL steps track the host rule
-    applied to the device's own maxima, and the state stays finite."""
-    n = 8192
-    cfg = Config(n=n, dim=2, nu=1e-5, dtype="float32")
+    """The 8192^2 fp32 workload under the device loop for 100 steps: CFL steps track the host
+    rule applied to the device's own maxima, the state stays finite, and the energy budget
+    of the dealiased equations holds: the nonlinear term moves energy between modes without
+    changing the total, so dE/dt = -2 nu Z (trapezoidal average of Z over the run).  A
+    dropped or mis-signed term in the step breaks the balance by orders of magnitude more
+    than the 2 % allowed for fp32 rounding of the energy over this interval."""
+    n, nu, steps = 8192, 1e-5, 100
+    cfg = Config(n=n, dim=2, nu=nu, dtype="float32")
     s = get("hip/fused").make(cfg)
     s.set_state(initial.random_phase_2d(n, k0=10.0, seed=0, dtype=np.complex64))
-    r = s.run_device(math.inf, cfl=0.5, dt_max=0.05, cfl_every=10, kmax=20)
-    assert r["steps"] == 20
+    d = s.diag
+    e0, z0 = d.energy(s.state_native()), d.enstrophy(s.state_native())
+    r = s.run_device(math.inf, cfl=0.5, dt_max=0.05, cfl_every=10, kmax=steps)
+    assert r["steps"] == steps
     dts = np.array(r["dt"])
     assert np.all(np.isfinite(dts)) and np.all(dts > 0) and np.all(dts <= 0.05)
     # dt is constant between controller updates
-    for k in range(20):
+    for k in range(steps):
         assert dts[k] == dts[k - k % 10]
-    umax = s.diag.max_velocity_sum(s.state_native())
+    umax = d.max_velocity_sum(s.state_native())
     assert math.isfinite(umax)
+    e1, z1 = d.energy(s.state_native()), d.enstrophy(s.state_native())
+    dedt = (e1 - e0) / dts.sum()
+    budget = -nu * (z0 + z1)
+    rel = abs(dedt - budget) / abs(budget)
+    print(f"  dE/dt {dedt:.6e} vs -2 nu Z {budget:.6e}: rel {rel:.2e}", flush=True)
+    assert rel < 2e-2
+
+
+def test_records_every_step_with_large_chunks():
+    """A record every step (diag_dt = dt) with chunks of 1500 steps enqueued 3 deep: 4500
+    records can land between two drains, more than a 4096-entry ring holds, so the record
+    ring is sized and bounded like the step ring.  Every record and step matches the host
+    rule's (the oracle's run)."""
+    cfg = _cfg(32, 2, "float64")
+    s = get("hip/fused").make(cfg)
+    ic = _ic(32, 2, seed=11)
+    s.set_state(ic)
+    t_end, dt = 50.0, 0.01
+    res = simulate.run(s, t_end, dt=dt, diag_dt=dt, device=True)
+    assert res.steps == 5000 and len(res.t) == 5001
+    # the same history through the solver's own loop, in large chunks
+    s.set_state(ic)
+    s.t, s.nstep = 0.0, 0
+    out = s.run_device(t_end, dt=dt, diag_dt=dt, chunk=1500, depth=3)
+    assert out["t"] == res.t and out["energy"] == res.energy and out["dt"] == res.dt
+    ref = O.OracleSolver(cfg)
+    ref.set_state(ic)
+    oref = O.run(ref, t_end, O.OracleDiagnostics(32, 2), dt=dt, diag_dt=dt)
+    assert len(oref.t) == len(out["t"])
+    np.testing.assert_allclose(out["t"], oref.t, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(out["energy"], oref.energy, rtol=1e-10)
+    np.testing.assert_allclose(out["enstrophy"], oref.enstrophy, rtol=1e-10)
+
+
+def test_continued_run_that_begins_finished_records_its_end():
+    """A run stopped early by its wall-clock limit (no end record), advanced to t_end by hand
+    and continued with the same history: the host loop records the end state because the last
+    record is older (simulate.py:93-95); the device loop must do the same."""
+    cfg = _cfg(32, 2, "float64")
+    ic = _ic(32, 2, seed=12)
+    hist = []
+    for device in (False, True):
+        s = get("hip/fused").make(cfg)
+        s.set_state(ic)
+        res = simulate.run(s, 0.1, dt=0.01, diag_dt=0.03, wall_limit=0.0, device=False)
+        assert not res.finished and res.t == [0.0]
+        s.advance(9, 0.01)
+        res = simulate.run(s, 0.1, dt=0.01, diag_dt=0.03, result=res, device=device)
+        hist.append(res)
+    host, dev = hist
+    assert dev.finished and host.finished
+    assert dev.t == host.t and len(dev.t) == 2
+    np.testing.assert_allclose(dev.energy, host.energy, rtol=1e-12)

@@ -9,6 +9,7 @@ plans; full-size runs are checked through exact solutions and invariants.
 from __future__ import annotations
 
 import math
+import os
 
 import numpy as np
 import pytest
@@ -179,3 +180,54 @@ def test_tgv2d_full_size_exact_decay():
     err = O.rel_err(got, exact)
     print(f"  8192^2 TGV rel err {err:.3e}", flush=True)
     assert err < 1e-5
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("k0", [10.0, 1000.0])
+def test_config2_8192_matches_oracle(k0):
+    """The benchmark's own workload at its full size, against the FP64 oracle over the whole
+    state: 8192^2 fp32, E(k) ~ k^4 exp(-(k/k0)^2) with random phases (seed 0), E = 0.5,
+    nu = 1e-5; two steps of the device controller (CFL 0.5).  The oracle replays the
+    controller's dt on the same masked input.
+    k0 = 10 is the benchmark's initial state; in complex64 its spectrum underflows to zero
+    beyond |k| ~ 110, so it holds about 40 k nonzero modes.  k0 = 1000 fills every one of the
+    14,913,991 retained modes but the mean, up to the last retained column (j = 2730) and the
+    mirrored negative-kx rows, with amplitudes that matter.
+    Tolerance: the 1e-5 relative bound set for the 8192^2 fp32 step."""
+    n, nu = 8192, 1e-5
+    print(f"case config-2 8192^2 fp32 k0={k0}", flush=True)
+    cfg = Config(n=n, dim=2, nu=nu, dtype="float32")
+    w0 = initial.random_phase_2d(n, k0=k0, seed=0, energy=0.5, dtype=np.complex64)
+    s = _make("hip/fused", cfg)
+    s.set_state(w0)
+    r = s.run_device(math.inf, cfl=0.5, dt_max=0.05, cfl_every=10, kmax=2, chunk=2, depth=1,
+                     use_graph=False, record_start=False)
+    dts = r["dt"]
+    sn = s.state_native()
+    e_gpu, z_gpu = s.diag.energy(sn), s.diag.enstrophy(sn)
+    got = s.get_state()
+    s.close()
+    del s, sn
+    h = O.OracleDiagnostics(n, 2)
+    w_in = O.OracleSolver(cfg.with_(dtype="float64")).check_state(w0)
+    dt_host = O.cfl_dt(h.max_velocity_sum(w_in), n, 0.5, 0.05)
+    print(f"  dt {dts} (host rule on the masked input {dt_host:.9e})", flush=True)
+    assert len(dts) == 2 and dts[0] == dts[1]
+    assert dts[0] == pytest.approx(dt_host, rel=1e-5)
+    ref = O.OracleSolver(cfg.with_(dtype="float64"), workers=min(os.cpu_count() or 1, 16))
+    ref.set_state(w0)
+    for h_ in dts:
+        ref.step(h_)
+    want = ref.get_state()
+    retained = int(np.count_nonzero(w_in))
+    err = O.rel_err(got, want)
+    e_ref, z_ref = h.energy(want), h.enstrophy(want)
+    print(f"  8192^2 config-2, 2 steps: rel err {err:.3e} over {retained} retained modes; "
+          f"E {abs(e_gpu - e_ref) / e_ref:.3e}, Z {abs(z_gpu - z_ref) / z_ref:.3e}", flush=True)
+    jm = (n - 1) // 3 + 1
+    if k0 > 100:
+        assert retained == (2 * jm - 1) * jm - 1 == 14_913_990  # all but the mean (k = 0)
+        assert np.abs(want[:, jm - 1]).max() > 1e-6 * np.abs(want).max()
+    assert err < 1e-5, err
+    assert abs(e_gpu - e_ref) < 1e-6 * e_ref
+    assert abs(z_gpu - z_ref) < 1e-6 * z_ref
