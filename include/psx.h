@@ -290,7 +290,10 @@ typedef struct psx_conv_args {
   int n, nb, relu;
 } psx_conv_args;
 int psx_cnn_conv(const psx_conv_args* a, void* stream);
-/* dw[27][32][32] += sum_p dz[p][o] x[p + tap][i]; db[32] += sum_p dz[p][o] (n = 64). */
+/* dw[27][32][32] += sum_p dz[p][o] x[p + tap][i]; db[32] += sum_p dz[p][o] (n = 64,
+ * 1 <= nb <= 4194303).  The per-work-group partial sums (512 x 27,680 floats, 56.7 MB)
+ * live in a buffer of this call's own, allocated and freed in stream order (hipMallocAsync /
+ * hipFreeAsync on `stream`), so calls on different streams do not share it. */
 int psx_cnn_wgrad(const void* x, const void* dz, float* dw, float* db, int n, int nb,
                   void* stream);
 /* g = 2 (y - t) / (npts nch) on the first nch channels (bf16 [npts][32]); loss += mean square. */

@@ -358,23 +358,34 @@ __global__ void __launch_bounds__(256) k_wgrad64(const __bf16* __restrict__ x,
 }
 
 constexpr int kWgGroups = 512;
+static_assert(kWg64Grid == kWgGroups, "one partial row per work-group of the gradient kernel");
+constexpr size_t kWgPartFloats = (size_t)kWgGroups * (kCnnTaps * kCnnC * kCnnC + kCnnC);
 
-hipError_t launch_conv_wgrad(const __bf16* x, const __bf16* dz, float* dw, float* db, int n,
-                             int nb, hipStream_t st) {
-  if (n != 64 || nb < 1) return hipErrorInvalidValue;
-  static_assert(kWg64Grid == kWgGroups, "one partial buffer serves both kernels");
-  static float* part = nullptr;  // per-process partial buffer (512 x 27.7k floats)
-  if (!part) {
-    hipError_t e = hipMalloc(&part, (size_t)kWgGroups * (kCnnTaps * kCnnC * kCnnC + kCnnC) *
-                                        sizeof(float));
-    if (e != hipSuccess) return e;
-  }
+// Both kernels of one weight gradient on the partial-sum buffer `part` (kWgPartFloats floats).
+// Every work-group of k_wgrad64 writes its whole partial row before k_wgrad_reduce reads it,
+// so the buffer needs no clearing between calls.
+static hipError_t enqueue_wgrad(const __bf16* x, const __bf16* dz, float* dw, float* db, int n,
+                                int nb, float* part, hipStream_t st) {
   const int nbricks = nb * (n * n * n / kCvPts);
   hipLaunchKernelGGL(k_wgrad64, dim3(kWg64Grid), dim3(256), kCvLds, st, x, dz, nbricks, part);
   const int nw = kCnnTaps * kCnnC * kCnnC + kCnnC;
   hipLaunchKernelGGL(k_wgrad_reduce, dim3((nw + kWgRedCols - 1) / kWgRedCols), dim3(256), 0,
                      st, part, kWgGroups, dw, db);
   return hipGetLastError();
+}
+
+hipError_t launch_conv_wgrad(const __bf16* x, const __bf16* dz, float* dw, float* db, int n,
+                             int nb, hipStream_t st) {
+  if (n != 64 || nb < 1) return hipErrorInvalidValue;
+  // a partial buffer of this call's own, allocated and freed in stream order: calls on other
+  // streams or threads never share it, and a captured call records it as graph nodes
+  float* part = nullptr;
+  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&part), kWgPartFloats * sizeof(float),
+                                st);
+  if (e != hipSuccess) return e;
+  e = enqueue_wgrad(x, dz, dw, db, n, nb, part, st);
+  const hipError_t ef = hipFreeAsync(part, st);
+  return e != hipSuccess ? e : ef;
 }
 
 // -------------------------------------------------------------------------------- loss

@@ -153,7 +153,9 @@ int psx_cnn_conv(const psx_conv_args* a, void* stream) {
 int psx_cnn_wgrad(const void* x, const void* dz, float* dw, float* db, int n, int nb,
                   void* stream) {
   if (!x || !dz || !dw || !db) return fail("psx_cnn_wgrad: null argument");
-  if (n != 64 || nb < 1) return fail("psx_cnn_wgrad: n must be 64 (got %d), nb >= 1", n);
+  if (n != 64) return fail("psx_cnn_wgrad: n must be 64 (got %d)", n);
+  if (nb < 1 || nb > kWgradMaxCubes)
+    return fail("psx_cnn_wgrad: nb = %d outside [1, %d]", nb, kWgradMaxCubes);
   PSX_HIP(launch_conv_wgrad(static_cast<const __bf16*>(x), static_cast<const __bf16*>(dz), dw,
                             db, n, nb, static_cast<hipStream_t>(stream)));
   return 0;

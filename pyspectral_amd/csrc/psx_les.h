@@ -67,8 +67,11 @@ struct ConvArgs {
 hipError_t launch_conv(const ConvArgs& a, hipStream_t st);
 
 // dW[tap][o][i] += sum_p dz[p][o] x[p + tap][i], db[o] += sum_p dz[p][o] (fp32, accumulated).
+// The per-work-group partial sums live in a buffer allocated and freed in stream order.
 hipError_t launch_conv_wgrad(const __bf16* x, const __bf16* dz, float* dw, float* db, int n,
                              int nb, hipStream_t st);
+// Cubes per weight-gradient call: nb * 512 bricks must fit the kernels' int brick index.
+constexpr int kWgradMaxCubes = (1 << 22) - 1;
 
 // Mean-square loss over the first `nch` channels: g = 2 (y - t) / count (bf16 [P][32],
 // zero elsewhere), loss (double, accumulated) = sum (y - t)^2 / count.

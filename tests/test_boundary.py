@@ -21,7 +21,8 @@ GOLD = ROOT / "tests" / "golden"
 
 def _header_symbols() -> list[str]:
     text = (ROOT / "include" / "psx.h").read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(psx_\w+)\s*\(", text, re.M)))
+    decl = r"^\s*(?:int|const char\*|unsigned long)\s+(psx_\w+)\s*\("
+    return sorted(set(re.findall(decl, text, re.M)))
 
 
 def test_header_and_binding_agree():

@@ -325,3 +325,30 @@ def test_planted_mistakes_are_caught(monkeypatch):
     monkeypatch.setattr(O.OracleSolver, "linear_operator",
                         lambda self, k2: orig_lin(self, k2) * 1.01)
     assert run_once() > 1e-6
+
+
+def test_bench_models_match_oracle_and_reference_row():
+    """bench.py's own traffic model (the product's measurement code may not import the oracle)
+    equals the oracle's restatement of benchmark.py:43-55 for every form, and its benchmark row
+    carries every key of the reference's row (benchmark.py:413-444)."""
+    import bench
+
+    for dim, form in ((2, "basdevant"), (2, "advective"), (3, "basdevant"), (3, "rotational")):
+        for dtype in ("float32", "float64"):
+            assert bench.traffic_model(96, dim, form, dtype) == O.traffic_model(96, dim, form,
+                                                                                dtype)
+    assert bench.traffic_model(8192, 2, "basdevant", "float32")["total_bytes"] == 20135477248.0
+    ceil = {"best_stage_s": 3e-3}
+    row = bench.bench_row("hip/fused", 64, 2, "basdevant", "float32", [1e-3, 1.1e-3, 0.9e-3],
+                          10, True, 0.1, 10.0, ceil, 5000.0, "x")
+    ref_keys = {"backend", "variant", "size", "dim", "dtype", "form", "metric", "value",
+                "repeats", "median", "min", "max", "step_ms", "step_ms_min", "step_ms_max",
+                "inner_steps", "bytes_per_unit", "effective_GBs", "pct_copy_bw", "pct_spec_bw",
+                "flops_per_unit", "pct_peak_flops", "ffts_per_step", "fft_Gpts_per_s",
+                "finite", "warmup_s", "peak_mem_MB", "notes", "fft_ceiling_step_ms",
+                "roofline_step_ms", "pct_fft_roofline", "fft_share_of_step"}
+    assert ref_keys <= set(row)
+    assert row["step_ms"] == pytest.approx(1.0) and row["repeats"] == 3
+    t_roof = 4 * 3e-3 + bench.traffic_model(64, 2, "basdevant", "float32")[
+        "pointwise_bytes"] / 5000e9
+    assert row["pct_fft_roofline"] == pytest.approx(100 * t_roof / 1e-3)
