@@ -291,11 +291,23 @@ typedef struct psx_conv_args {
 } psx_conv_args;
 int psx_cnn_conv(const psx_conv_args* a, void* stream);
 /* dw[27][32][32] += sum_p dz[p][o] x[p + tap][i]; db[32] += sum_p dz[p][o] (n = 64,
- * 1 <= nb <= 4194303).  The per-work-group partial sums (512 x 27,680 floats, 56.7 MB)
- * live in a buffer of this call's own, allocated and freed in stream order (hipMallocAsync /
- * hipFreeAsync on `stream`), so calls on different streams do not share it. */
+ * 1 <= nb <= 4194303).  The per-work-group partial sums (56.7 MB,
+ * psx_cnn_wgrad_scratch_bytes()) live in a buffer of this call's own, allocated and freed in
+ * stream order (hipMallocAsync / hipFreeAsync on `stream`), so calls on different streams do
+ * not share it. */
 int psx_cnn_wgrad(const void* x, const void* dz, float* dw, float* db, int n, int nb,
                   void* stream);
+/* The same weight gradient with no allocation, bit for bit the result of psx_cnn_wgrad.  No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight training or serving loops, harnesses that forbid
+ * hipMalloc).  The partial sums go to a static array of psx_cnn_wgrad_scratch_bytes() bytes
+ * (56.7 MB) per GPU, held until the process ends; its size is fixed by the kernel's 512
+ * work-groups, not by nb, and nb is limited as for psx_cnn_wgrad (nb > 4194303 is rejected
+ * with an error).  Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (other streams, other threads) race and must be serialised by the caller. */
+int psx_cnn_wgrad_noalloc(const void* x, const void* dz, float* dw, float* db, int n, int nb,
+                          void* stream);
+unsigned long psx_cnn_wgrad_scratch_bytes(void);
 /* g = 2 (y - t) / (npts nch) on the first nch channels (bf16 [npts][32]); loss += mean square. */
 int psx_cnn_mse(const float* y, const float* t, int tstride, int nch, long long npts, void* g,
                 double* loss, void* stream);

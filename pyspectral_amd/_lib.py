@@ -118,6 +118,9 @@ SIGNATURES = {
     "psx_les_op": (c_int, [c_int, POINTER(LesArgs), c_void_p]),
     "psx_cnn_conv": (c_int, [POINTER(ConvArgs), c_void_p]),
     "psx_cnn_wgrad": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    "psx_cnn_wgrad_noalloc": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+                                      c_void_p]),
+    "psx_cnn_wgrad_scratch_bytes": (ctypes.c_ulong, []),
     "psx_cnn_mse": (c_int, [c_void_p, c_void_p, c_int, c_int, c_longlong, c_void_p, c_void_p,
                             c_void_p]),
     "psx_cnn_adam": (c_int, [POINTER(AdamArgs), c_void_p]),

@@ -6,6 +6,8 @@
 // row (l & 15) of A and column (l & 15) of B, each with 8 consecutive k of k-group (l >> 4),
 // and D[4 (l >> 4) + j][l & 15], j < 4.  Which k a lane's element stands for only has to
 // agree between A and B, so each kernel chooses its own k order.
+#include <mutex>
+
 #include "psx_kernels.h"
 #include "psx_les.h"
 
@@ -386,6 +388,42 @@ hipError_t launch_conv_wgrad(const __bf16* x, const __bf16* dz, float* dw, float
   e = enqueue_wgrad(x, dz, dw, db, n, nb, part, st);
   const hipError_t ef = hipFreeAsync(part, st);
   return e != hipSuccess ? e : ef;
+}
+
+// Partial sums of psx_cnn_wgrad_noalloc: one array per GPU, held until the process ends.  Host
+// code never passes the array's name to a kernel (on the host it is only a stand-in): the
+// device address comes from hipGetSymbolAddress once per device, with that device current.
+static __device__ __align__(256) float scratch[kWgPartFloats];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static float* g_scratch_ptr[kMaxDevices] = {};
+
+static hipError_t scratch_ptr(float** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
+size_t wgrad_scratch_bytes() { return sizeof(float) * kWgPartFloats; }
+
+hipError_t launch_conv_wgrad_noalloc(const __bf16* x, const __bf16* dz, float* dw, float* db,
+                                     int n, int nb, hipStream_t st) {
+  if (n != 64 || nb < 1) return hipErrorInvalidValue;
+  float* part = nullptr;
+  const hipError_t e = scratch_ptr(&part);
+  if (e != hipSuccess) return e;
+  return enqueue_wgrad(x, dz, dw, db, n, nb, part, st);
 }
 
 // -------------------------------------------------------------------------------- loss

@@ -161,6 +161,20 @@ int psx_cnn_wgrad(const void* x, const void* dz, float* dw, float* db, int n, in
   return 0;
 }
 
+int psx_cnn_wgrad_noalloc(const void* x, const void* dz, float* dw, float* db, int n, int nb,
+                          void* stream) {
+  if (!x || !dz || !dw || !db) return fail("psx_cnn_wgrad_noalloc: null argument");
+  if (n != 64) return fail("psx_cnn_wgrad_noalloc: n must be 64 (got %d)", n);
+  if (nb < 1 || nb > kWgradMaxCubes)
+    return fail("psx_cnn_wgrad_noalloc: nb = %d outside [1, %d]", nb, kWgradMaxCubes);
+  PSX_HIP(launch_conv_wgrad_noalloc(static_cast<const __bf16*>(x),
+                                    static_cast<const __bf16*>(dz), dw, db, n, nb,
+                                    static_cast<hipStream_t>(stream)));
+  return 0;
+}
+
+unsigned long psx_cnn_wgrad_scratch_bytes(void) { return wgrad_scratch_bytes(); }
+
 int psx_cnn_mse(const float* y, const float* t, int tstride, int nch, long long npts, void* g,
                 double* loss, void* stream) {
   if (!y || !t || !g || !loss || nch < 1 || nch > 32 || tstride < nch || npts < 1)

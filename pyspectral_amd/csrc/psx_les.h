@@ -70,6 +70,11 @@ hipError_t launch_conv(const ConvArgs& a, hipStream_t st);
 // The per-work-group partial sums live in a buffer allocated and freed in stream order.
 hipError_t launch_conv_wgrad(const __bf16* x, const __bf16* dz, float* dw, float* db, int n,
                              int nb, hipStream_t st);
+// The same kernels on the library's static per-GPU partial-sum array (no allocation; calls
+// that overlap on one GPU race on it).
+hipError_t launch_conv_wgrad_noalloc(const __bf16* x, const __bf16* dz, float* dw, float* db,
+                                     int n, int nb, hipStream_t st);
+size_t wgrad_scratch_bytes();
 // Cubes per weight-gradient call: nb * 512 bricks must fit the kernels' int brick index.
 constexpr int kWgradMaxCubes = (1 << 22) - 1;
 
