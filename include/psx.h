@@ -40,6 +40,8 @@ typedef struct psx_flow_params {
   double ka2;  /* drag band upper k^2 (1e30: uniform drag) */
   double klo2; /* forcing band lower k^2 */
   double khi2; /* forcing band upper k^2 */
+  int dealias; /* 0: 2/3 rule (cubic, 3 |k_i| < n); 1: phase shift with the spherical mask
+                  |k|^2 < 2 (n/3)^2 (config.py:38, grid.py:36-50, numpy_ref.py:68-72,125-153) */
 } psx_flow_params;
 
 /* Buffer sizes in complex elements of the working precision. */
@@ -79,6 +81,17 @@ int psx_prepare(psx_ctx* c, void* stream);
  * kernels read, so a captured step graph stays valid.  Replaces cupy_cuda.py:151-161
  * (_update_table) and src/pyspectral/backends/_fused_common.py:10-19 (if_table, if_scalars). */
 int psx_set_step(psx_ctx* c, double dt, double amp, void* stream);
+
+/* Phase-shift dealiasing (dealias = 1): the grid shifts of the next step.  shift[d] is the
+ * random shift the reference's oracle draws for that step (rng.uniform(0, 2 pi / n), one per
+ * axis, numpy_ref.py:125-128); stages 1 and 2 use it, stages 3 and 4 shift + pi / n
+ * (numpy_ref.py:130-132, added here in FP64 as there); next[d] is the shift of the step after
+ * it, applied to the inputs the last stage prepares.  The prepare call uses shift.  Every
+ * stage multiplies its input state by exp(+i k.shift) and its nonlinear term by
+ * exp(-i k.shift) (FP64 phases, rounded once).  The values live in the device parameter
+ * block, so a captured step graph stays valid.  Replaces numpy_ref.py:68-72,125-153 (the
+ * reference's CUDA variant has no phase shift). */
+int psx_set_phase_shift(psx_ctx* c, const double* shift, const double* next, void* stream);
 
 /* One IF-RK4 step (4 stages) with the current parameters.  Replaces cupy_cuda.py:163-195
  * (_stage, _step without graph). */

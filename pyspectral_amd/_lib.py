@@ -16,7 +16,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libpsx.so"
 class FlowParams(ctypes.Structure):
     _fields_ = [("n", c_int), ("dim", c_int), ("form", c_int), ("fp64", c_int),
                 ("nu", c_double), ("alpha", c_double), ("ka2", c_double), ("klo2", c_double),
-                ("khi2", c_double)]
+                ("khi2", c_double), ("dealias", c_int)]
 
 
 class Layout(ctypes.Structure):
@@ -81,6 +81,7 @@ SIGNATURES = {
     "psx_destroy": (c_int, [c_void_p]),
     "psx_prepare": (c_int, [c_void_p, c_void_p]),
     "psx_set_step": (c_int, [c_void_p, c_double, c_double, c_void_p]),
+    "psx_set_phase_shift": (c_int, [c_void_p, POINTER(c_double), POINTER(c_double), c_void_p]),
     "psx_step": (c_int, [c_void_p, c_void_p]),
     "psx_step_graph": (c_int, [c_void_p, c_void_p]),
     "psx_velocity_max": (c_int, [c_void_p, POINTER(c_double), c_void_p]),

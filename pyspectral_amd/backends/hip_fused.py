@@ -26,7 +26,8 @@ _FORM_CODE = {"basdevant": 0, "rotational": 1}
 def flow_params(cfg: Config, form: str) -> _lib.FlowParams:
     return _lib.FlowParams(n=cfg.n, dim=cfg.dim, form=_FORM_CODE[form], fp64=int(cfg.is_fp64),
                            nu=cfg.nu, alpha=cfg.alpha, ka2=cfg.drag_k2,
-                           klo2=cfg.force_klo**2, khi2=cfg.force_khi**2)
+                           klo2=cfg.force_klo**2, khi2=cfg.force_khi**2,
+                           dealias=1 if cfg.dealias == "phase-shift" else 0)
 
 
 def check_ring_bounds(chunk: int, depth: int, cap_step: int, cap_rec: int) -> None:
@@ -87,12 +88,21 @@ class HipSolver(SpectralSolver):
     variant = "fused"
     default_form = "basdevant"
     forms = ("basdevant", "rotational")
+    phase_shift = True
     device_loop = True  # run_device: the time loop of simulate.run on the GPU
 
-    def __init__(self, cfg: Config, graph: bool = False) -> None:
+    def __init__(self, cfg: Config, graph: bool = False, seed: int = 1234) -> None:
         super().__init__(cfg)
         if self.dim == 2 and self.form != "basdevant":
             raise BackendUnavailable("hip: 2D supports the basdevant form only")
+        # phase-shift dealiasing: the grid shifts come from the oracle's generator and seed
+        # (numpy_ref.py:46,125-128), drawn one step ahead on the host; the device-resident
+        # loop has no generator of its own, so such solvers step from the host
+        self.ps = cfg.dealias == "phase-shift"
+        self.rng = np.random.default_rng(seed)
+        self._shift: np.ndarray | None = None  # the next step's shift, once drawn
+        if self.ps:
+            self.device_loop = False
         ok, why = self.available()
         if not ok:
             raise BackendUnavailable(f"hip: {why}")
@@ -135,6 +145,7 @@ class HipSolver(SpectralSolver):
                                               else np.complex64))
         self.buf.u0.copy_(host.to(self.buf.u0.device))
         self.buf.acc.zero_()
+        self._set_shifts(self._next_shift(), None)
         _lib.check(self.lib.psx_prepare(self.ctx, _lib.stream_ptr()))
 
     def set_state_device(self, state: Any) -> None:
@@ -143,9 +154,30 @@ class HipSolver(SpectralSolver):
             raise ValueError(f"state shape {tuple(state.shape)} != {self.state_shape()}")
         self.buf.u0.copy_(state.to(self.cdt))
         self._mask_device(self.buf.u0)
+        self._set_shifts(self._next_shift(), None)
         _lib.check(self.lib.psx_prepare(self.ctx, _lib.stream_ptr()))
 
+    def _next_shift(self) -> np.ndarray | None:
+        """The shift of the next step (drawn now if not yet drawn), as numpy_ref.py:125-128."""
+        if not self.ps:
+            return None
+        if self._shift is None:
+            self._shift = self.rng.uniform(0.0, 2.0 * np.pi / self.n, size=self.dim)
+        return self._shift
+
+    def _set_shifts(self, cur: np.ndarray | None, nxt: np.ndarray | None) -> None:
+        if cur is None:
+            return
+        nxt = cur if nxt is None else nxt
+        a = (ctypes.c_double * 3)(*cur)
+        b = (ctypes.c_double * 3)(*nxt)
+        _lib.check(self.lib.psx_set_phase_shift(self.ctx, a, b, _lib.stream_ptr()))
+
     def _mask_device(self, t: Any) -> None:
+        if self.ps:
+            m = self.torch.from_numpy(self.mask_np()).to(t.device)
+            t.mul_(m if self.dim == 2 else m.unsqueeze(0))
+            return
         n, jm = self.n, retained(self.n)
         lo, hi = jm, n - jm + 1  # full-axis indices jm .. n-jm are outside 3|k| < n
         t[..., jm:] = 0
@@ -165,6 +197,11 @@ class HipSolver(SpectralSolver):
         if key != self._key:
             _lib.check(self.lib.psx_set_step(self.ctx, key[0], key[1], st))
             self._key = key
+        if self.ps:
+            cur = self._next_shift()
+            self._shift = None
+            nxt = self._next_shift()  # drawn now: the last stage prepares its inputs
+            self._set_shifts(cur, nxt)
         if self.graph:
             _lib.check(self.lib.psx_step_graph(self.ctx, st))
         else:
@@ -183,6 +220,9 @@ class HipSolver(SpectralSolver):
         ``last_record``: time of the last record of an earlier part of the same history (a
         continued run records its end state only when that one is older).
         Returns the records (t, E, Z) and the step history (h, A)."""
+        if self.ps:
+            raise BackendUnavailable("hip: the device-resident loop runs the 2/3 rule only "
+                                     "(phase-shift grids are drawn on the host); use step()")
         lib, st = self.lib, _lib.stream_ptr()
         chunk = int(chunk or cfl_every)
         # a step writes one step entry and at most one record: both rings take the same bound
@@ -260,5 +300,5 @@ class HipGraphSolver(HipSolver):
 
     variant = "fused-graph"
 
-    def __init__(self, cfg: Config) -> None:
-        super().__init__(cfg, graph=True)
+    def __init__(self, cfg: Config, seed: int = 1234) -> None:
+        super().__init__(cfg, graph=True, seed=seed)

@@ -179,10 +179,14 @@ int plan_flow(const psx_flow_params& p, psx_ctx* c, Flow<T>* f) {
   if (p.dim != 2 && p.dim != 3) return fail("psx: dim must be 2 or 3");
   if (p.dim == 2 && p.form != 0) return fail("psx: 2D supports the basdevant form only");
   if (p.form != 0 && p.form != 1) return fail("psx: form must be 0 or 1");
+  if (p.dealias != 0 && p.dealias != 1)
+    return fail("psx: dealias must be 0 (2/3 rule) or 1 (phase shift)");
   std::memset(f, 0, sizeof *f);
   f->n = n;
   f->nh = n / 2 + 1;
-  f->jm = (n - 1) / 3 + 1;
+  f->ps = p.dealias == 1;
+  // retained half-axis columns: 3 j < n, or 9 j^2 < 2 n^2 for the phase shift's disc/ball
+  f->jm = f->ps ? ct_retained_ps(n) : ct_retained(n);
   f->dim = p.dim;
   f->rot = p.dim == 3 && p.form == 1;
   f->scale = (T)(1.0 / std::pow((double)n, p.dim));
@@ -203,7 +207,7 @@ int plan_flow(const psx_flow_params& p, psx_ctx* c, Flow<T>* f) {
   const int nmax = p.dim == 2 ? 2 : 3;
   if (p.dim == 2) {
     // one planner for the run-time and the compile-time specialised kernels
-    const Geo2D g = geo2d(n, fp64);
+    const Geo2D g = geo2d(n, fp64, f->jm);
     if (!g.ok) return fail("psx: no 2D plan for n=%d", n);
     auto conv = [](const CtPlan& cp, FftPlan* fp) {
       fp->L = cp.L;
@@ -238,8 +242,9 @@ int plan_flow(const psx_flow_params& p, psx_ctx* c, Flow<T>* f) {
       c->outerF.lds = (size_t)g.n1 * g.colT * csz;
       c->outerI = c->outerF;
     }
-    c->ct = !fp64 && ct2d_supported(n);
-  } else if (!fp64 && ct3d_supported(n)) {
+    // the phase shift runs on the run-time kernels (the specialised ones assume the 2/3 rule)
+    c->ct = !fp64 && !f->ps && ct2d_supported(n);
+  } else if (!fp64 && !f->ps && ct3d_supported(n)) {
     // compile-time specialised 3D kernels: plans from the shared planner
     const Geo3D g = geo3d(n);
     if (!g.ok) return fail("psx: no 3D plan for n=%d", n);
@@ -542,8 +547,10 @@ int psx_create(const psx_flow_params* p, void* u0, void* acc, void* work, psx_ct
   }
   // default parameters: dt = 0, no forcing (the caller sets them before stepping)
   hipStream_t st = nullptr;
+  const double zero9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // phase shifts: unshifted grids
   if (launch_params(c->prm, nh, p->nu, 0.0, p->alpha, 0.0, p->klo2, p->khi2, p->ka2, st) !=
           hipSuccess ||
+      launch_set_f64(c->prm + 2 * nh + PrmLayout::kPsA, zero9, 9, st) != hipSuccess ||
       hipStreamSynchronize(st) != hipSuccess) {
     psx_destroy(c);
     return fail("psx_create: parameter kernel failed");
@@ -602,6 +609,28 @@ int psx_set_step(psx_ctx* c, double dt, double amp, void* stream) {
   const int nh = c->p.n / 2 + 1;
   PSX_HIP(launch_params(c->prm, nh, c->p.nu, dt, c->p.alpha, amp, c->p.klo2, c->p.khi2,
                         c->p.ka2, st));
+  return 0;
+}
+
+int psx_set_phase_shift(psx_ctx* c, const double* shift, const double* next, void* stream) {
+  if (!c || !shift || !next) return fail("psx_set_phase_shift: null argument");
+  if (c->p.dealias != 1) return fail("psx_set_phase_shift: the context uses the 2/3 rule");
+  if (c->run_mode) return fail("psx_set_phase_shift: a device-loop run is active");
+  const int d = c->p.dim;
+  const double half = M_PI / c->p.n;  // the oracle's shift + pi / n, in FP64
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < d; ++i) {
+    if (!std::isfinite(shift[i]) || !std::isfinite(next[i]))
+      return fail("psx_set_phase_shift: non-finite shift");
+    v[i] = shift[i];
+    v[3 + i] = shift[i] + half;
+    v[6 + i] = next[i];
+  }
+  const int nh = c->p.n / 2 + 1;
+  static_assert(PrmLayout::kPsB == PrmLayout::kPsA + 3 && PrmLayout::kPsN == PrmLayout::kPsA + 6,
+                "shift entries are contiguous");
+  PSX_HIP(launch_set_f64(c->prm + 2 * nh + PrmLayout::kPsA, v, 9,
+                         static_cast<hipStream_t>(stream)));
   return 0;
 }
 
@@ -710,6 +739,7 @@ int psx_spectrum(psx_ctx* c, const void* state, int pitch, double* out, int nshe
 int psx_set_source(psx_ctx* c, void* src) {
   if (!c) return fail("psx_set_source: null context");
   if (c->p.dim != 3) return fail("psx_set_source: 3D flows only");
+  if (src && c->p.dealias != 0) return fail("psx_set_source: 2/3-rule flows only");
   if (c->fp64) c->f64.src = static_cast<const cx<double>*>(src);
   else c->f32.src = static_cast<const cx<float>*>(src);
   // the flow descriptor is a kernel argument: captured graphs hold the old one
@@ -763,6 +793,9 @@ int psx_sgs_update(psx_ctx* c, void* stream) {
 int psx_run_begin(psx_ctx* c, const psx_run_params* rp, void* stream) {
   if (!c || !rp) return fail("psx_run_begin: null argument");
   if (rp->cfl_every < 1) return fail("psx_run_begin: cfl_every must be >= 1");
+  if (c->p.dealias != 0)
+    return fail("psx_run_begin: phase-shift flows draw their shifts on the host; step them "
+                "with psx_step / psx_step_graph");
   if (rp->cap_rec < 1 || rp->cap_step < 1) return fail("psx_run_begin: ring capacities < 1");
   if (!(rp->dt_fixed > 0.0) && !(rp->cfl > 0.0 && rp->dt_max > 0.0))
     return fail("psx_run_begin: need dt_fixed > 0, or cfl > 0 and dt_max > 0");

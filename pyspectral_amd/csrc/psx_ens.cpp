@@ -43,8 +43,8 @@ Geo2D ens_geo(int n, int var) {
 }
 
 int ens_plan(const psx_flow_params& p, int var, Geo2D* g, psx_layout* lay) {
-  if (p.dim != 2 || p.form != 0 || p.fp64)
-    return fail("psx_ens: members are 2D basdevant fp32 flows");
+  if (p.dim != 2 || p.form != 0 || p.fp64 || p.dealias != 0)
+    return fail("psx_ens: members are 2D basdevant fp32 flows with the 2/3 rule");
   if (var < 0) var = default_variant(p.n);
   if (var < 0 || !ens_compiled(p.n, var))
     return fail("psx_ens: no compiled ensemble kernels for n=%d variant %d", p.n, var);

@@ -20,9 +20,13 @@
 namespace psx {
 
 // FP64 parameter block: [0, nh) exp(-nu m^2 dt), [nh, 2 nh) exp(-nu m^2 dt/2), then scalars.
+// Phase shift (Flow::ps): kPsA the step's grid shift (3 entries: stages 0-1 and the inputs of
+// stages 0-1), kPsB that shift + pi/n (stages 2-3 and their inputs), kPsN the next step's shift
+// (inputs prepared by stage 3).
 struct PrmLayout {
   static constexpr int kDt = 0, kEa1 = 1, kEah = 2, kEb1 = 3, kEbh = 4, kKlo2 = 5, kKhi2 = 6,
-                       kKa2 = 7, kBand = 8, kActive = 9, kNScalars = 16;
+                       kKa2 = 7, kBand = 8, kActive = 9, kPsA = 16, kPsB = 19, kPsN = 22,
+                       kNScalars = 32;
 };
 
 template <typename T>
@@ -41,6 +45,7 @@ struct Flow {
   int n, nh, jm, ldw;
   int sp;            // state row pitch (complex elements)
   int dim, rot;      // rot: 3D rotational form
+  int ps;            // phase-shift dealiasing: spherical mask 9 |k|^2 < 2 n^2, shifted grids
   int n1, n2;        // four-step split of the full axes (n1 = 1: single pass)
   int rowT;          // rows per work-group in the line kernel
   int colT;          // columns per work-group in the column kernels

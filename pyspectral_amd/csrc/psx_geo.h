@@ -85,6 +85,12 @@ constexpr int ct_state_pitch(int n, bool fp64) {
   return (n / 2 + 1 + q - 1) / q * q;
 }
 constexpr int ct_retained(int n) { return (n - 1) / 3 + 1; }
+// Retained half-axis width of the spherical phase-shift mask: 9 j^2 < 2 n^2 for j < jm.
+constexpr int ct_retained_ps(int n) {
+  int j = 0;
+  while (9LL * j * j < 2LL * n * n) ++j;
+  return j;
+}
 
 // Launch geometry of the 2D kernels for grid size n in fp32 (csz = 8) or fp64 (csz = 16).
 struct Geo2D {
@@ -103,11 +109,11 @@ constexpr int ct_pick_E(int L, bool fp64) {
   return 16;
 }
 
-constexpr Geo2D geo2d(int n, bool fp64) {
+constexpr Geo2D geo2d(int n, bool fp64, int jm = 0) {
   Geo2D g{};
   const int csz = fp64 ? 16 : 8;
   g.n = n;
-  g.jm = ct_retained(n);
+  g.jm = jm > 0 ? jm : ct_retained(n);
   g.sp = ct_state_pitch(n, fp64);
   int T = fp64 ? 8 : 16;
   while (T > 1 && T / 2 >= g.jm) T /= 2;

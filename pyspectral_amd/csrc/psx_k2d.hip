@@ -154,21 +154,25 @@ __global__ void __launch_bounds__(512, 1) k_spec2d(Flow<T> f, int stage) {
       const int ky = j0 + t;
       const int ax = kx < 0 ? -kx : kx;
       cx<T> ou = czero<T>(), ov = czero<T>();
-      if (3 * ax < n && ky < f.jm) {
+      const long long k2 = (long long)kx * kx + (long long)ky * ky;
+      // retained modes: the 2/3 rule's square, or the phase shift's disc 9 |k|^2 < 2 n^2
+      const bool keep = f.ps ? 9 * k2 < 2LL * n * n : (3 * ax < n && ky < f.jm);
+      if (keep) {
         const long long si = (long long)k * f.sp + ky;
-        const long long k2 = (long long)kx * kx + (long long)ky * ky;
         cx<T> nxt;
         if (stage == kStagePre) {
           nxt = f.u0[si];
         } else {
           const T fx = (T)kx, fy = (T)ky;
-          const cx<T> N = cadd(cscl(fx * fy, va[i]), cscl(fx * fx - fy * fy, vb[i]));
+          cx<T> N = cadd(cscl(fx * fy, va[i]), cscl(fx * fx - fy * fy, vb[i]));
+          if (f.ps) N = cmul(N, shift_phase<T>(shift_of_term(f.prm, nh, stage), kx, ky, 0, -1.0));
           double E1, Eh;
           if_factors(f.prm, nh, s, ax, ky, -1, k2, E1, Eh);
-          nxt = rk_update<T>(stage, dt, (T)E1, (T)Eh, N, f.u0 + si, f.acc + si);
+          nxt = rk_update<T>(stage, dt, (T)(1.0 - E1), (T)(1.0 - Eh), N, f.u0 + si, f.acc + si);
         }
         const T ik2 = k2 > 0 ? f.scale / (T)k2 : (T)0;
-        const cx<T> psi = cscl(ik2, nxt);
+        cx<T> psi = cscl(ik2, nxt);
+        if (f.ps) psi = cmul(psi, shift_phase<T>(shift_of_inputs(f.prm, nh, stage), kx, ky, 0, 1.0));
         ou = cscl((T)ky, mul_i(psi));
         ov = cscl((T)kx, mul_mi(psi));
       }

@@ -202,7 +202,7 @@ __device__ __forceinline__ void body_spec2d(const Flow<float>& f, int tile, int 
         } else if constexpr (stage >= 1) {
           acc = ab.load(soff[i]);
         }
-        nxt = rk_value<stage, T>(dt, (T)E1, (T)Eh, Nl, uu[i], acc);
+        nxt = rk_value<stage, T>(dt, (T)(1.0 - E1), (T)(1.0 - Eh), Nl, uu[i], acc);
         if constexpr (stage < 3) ab.store(soff[i], acc);
         else ub.store(soff[i], nxt);
       }

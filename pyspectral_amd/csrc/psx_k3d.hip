@@ -309,9 +309,11 @@ __global__ void __launch_bounds__(512, 1) k_spec3d(Flow<T> f, int stage) {
       const int ax = kx < 0 ? -kx : kx;
       const int kz = j0 + t;
       cx<T> ua = czero<T>(), ub = czero<T>(), uc = czero<T>();
-      if (3 * ax < n && 3 * ay < n && kz < f.jm) {
+      const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
+      // retained modes: the 2/3 rule's cube, or the phase shift's ball 9 |k|^2 < 2 n^2
+      const bool keep = f.ps ? 9 * k2 < 2LL * n * n : (3 * ax < n && 3 * ay < n && kz < f.jm);
+      if (keep) {
         const long long si = ((long long)e * n + iy) * f.sp + kz;
-        const long long k2 = (long long)kx * kx + (long long)ky * ky + (long long)kz * kz;
         if (stage == kStagePre) {
           ua = f.u0[si];
           ub = f.u0[sst + si];
@@ -329,15 +331,21 @@ __global__ void __launch_bounds__(512, 1) k_spec3d(Flow<T> f, int stage) {
             a1[i] = cadd(a1[i], f.src[sst + si]);
             a2[i] = cadd(a2[i], f.src[2 * sst + si]);
           }
+          if (f.ps) {  // back from the shifted grid (the projection below commutes with it)
+            const cx<T> ph = shift_phase<T>(shift_of_term(f.prm, nh, stage), kx, ky, kz, -1.0);
+            a0[i] = cmul(a0[i], ph);
+            a1[i] = cmul(a1[i], ph);
+            a2[i] = cmul(a2[i], ph);
+          }
           const cx<T> d = cscl(ik2, cadd(cadd(cscl(fx, a0[i]), cscl(fy, a1[i])), cscl(fz, a2[i])));
           const cx<T> n0 = csub(a0[i], cscl(fx, d));
           const cx<T> n1 = csub(a1[i], cscl(fy, d));
           const cx<T> n2 = csub(a2[i], cscl(fz, d));
           double E1, Eh;
           if_factors(f.prm, nh, s, ax, ay, kz, k2, E1, Eh);
-          ua = rk_update<T>(stage, dt, (T)E1, (T)Eh, n0, f.u0 + si, f.acc + si);
-          ub = rk_update<T>(stage, dt, (T)E1, (T)Eh, n1, f.u0 + sst + si, f.acc + sst + si);
-          uc = rk_update<T>(stage, dt, (T)E1, (T)Eh, n2, f.u0 + 2 * sst + si,
+          ua = rk_update<T>(stage, dt, (T)(1.0 - E1), (T)(1.0 - Eh), n0, f.u0 + si, f.acc + si);
+          ub = rk_update<T>(stage, dt, (T)(1.0 - E1), (T)(1.0 - Eh), n1, f.u0 + sst + si, f.acc + sst + si);
+          uc = rk_update<T>(stage, dt, (T)(1.0 - E1), (T)(1.0 - Eh), n2, f.u0 + 2 * sst + si,
                             f.acc + 2 * sst + si);
           if (stage == 3) {  // the new state, projected (rk_update stored it unprojected)
             leray<T>(fx, fy, fz, ik2, ua, ub, uc);
@@ -349,6 +357,12 @@ __global__ void __launch_bounds__(512, 1) k_spec3d(Flow<T> f, int stage) {
         ua = cscl(f.scale, ua);
         ub = cscl(f.scale, ub);
         uc = cscl(f.scale, uc);
+        if (f.ps) {  // the next stage's inputs on its shifted grid
+          const cx<T> ph = shift_phase<T>(shift_of_inputs(f.prm, nh, stage), kx, ky, kz, 1.0);
+          ua = cmul(ua, ph);
+          ub = cmul(ub, ph);
+          uc = cmul(uc, ph);
+        }
       }
       a0[i] = ua;
       a1[i] = ub;

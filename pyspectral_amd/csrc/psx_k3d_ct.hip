@@ -419,9 +419,9 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
           c1 = ab.load(so(i, 1));
           c2 = ab.load(so(i, 2));
         }
-        ua = rk_value<stage, T>(dt, (T)E1, (T)Eh, n0, u0a, c0);
-        ubv = rk_value<stage, T>(dt, (T)E1, (T)Eh, n1, u0b, c1);
-        uc = rk_value<stage, T>(dt, (T)E1, (T)Eh, n2, u0c, c2);
+        ua = rk_value<stage, T>(dt, (T)(1.0 - E1), (T)(1.0 - Eh), n0, u0a, c0);
+        ubv = rk_value<stage, T>(dt, (T)(1.0 - E1), (T)(1.0 - Eh), n1, u0b, c1);
+        uc = rk_value<stage, T>(dt, (T)(1.0 - E1), (T)(1.0 - Eh), n2, u0c, c2);
         if constexpr (stage < 3) {
           ab.store(so(i, 0), c0);
           ab.store(so(i, 1), c1);

@@ -72,6 +72,9 @@ hipError_t launch_hermitian(const Flow<T>& f, hipStream_t st);
 hipError_t launch_params(double* prm, int nh, double nu, double dt, double alpha, double amp,
                          double klo2, double khi2, double ka2, hipStream_t st);
 hipError_t launch_fill_u32(unsigned* p, unsigned v, int n, hipStream_t st);
+// Write n <= 16 doubles (kernel arguments) to device memory: a kernel, so captured graphs
+// and the parameter block's other writers stay ordered with it on the stream.
+hipError_t launch_set_f64(double* p, const double* v, int n, hipStream_t st);
 hipError_t launch_fill_f64(double* p, double v, int n, hipStream_t st);
 template <typename T>
 hipError_t launch_reduce(const cx<T>* state, long long sstride, int pitch, int n, int dim,

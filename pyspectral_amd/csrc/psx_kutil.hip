@@ -30,6 +30,21 @@ __global__ void k_fill_f64(double* p, double v, int n) {
   if (i < n) p[i] = v;
 }
 
+struct F64x16 {
+  double v[16];
+};
+__global__ void k_set_f64(double* p, F64x16 v, int n) {
+  const int i = threadIdx.x;
+  if (i < n) p[i] = v.v[i];
+}
+hipError_t launch_set_f64(double* p, const double* v, int n, hipStream_t st) {
+  if (n < 1 || n > 16) return hipErrorInvalidValue;
+  F64x16 a{};
+  for (int i = 0; i < n; ++i) a.v[i] = v[i];
+  hipLaunchKernelGGL(k_set_f64, dim3(1), dim3(16), 0, st, p, a, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_fill_u32(unsigned* p, unsigned v, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, st, p, v, n);
   return hipGetLastError();

@@ -191,3 +191,23 @@ def test_device_loop_ring_bounds():
         check_ring_bounds(342, 3, 1 << 12, 1 << 10)
     with pytest.raises(ValueError):
         check_ring_bounds(0, 3, 1 << 14, 1 << 14)
+
+
+@pytest.mark.skipif(not _lib.LIB_PATH.exists(), reason="libpsx.so not built")
+@pytest.mark.parametrize("n", [8, 16, 32, 48, 96, 256, 1024, 8192])
+@pytest.mark.parametrize("dim,fp64", [(2, 0), (2, 1), (3, 0), (3, 1)])
+def test_phase_shift_layout_holds_the_spherical_mask(n, dim, fp64):
+    """dealias = 1: the stored half-axis columns are exactly those the spherical mask
+    |k|^2 < 2 (n/3)^2 of the reference (grid.py:36-50) can reach."""
+    if dim == 3 and n > 512:
+        pytest.skip("3D grids above 512 are not planned")
+    lib = _lib.load()
+    p = _lib.FlowParams(n=n, dim=dim, form=0, fp64=fp64, nu=1e-3, alpha=0.0, ka2=1e30,
+                        klo2=0.0, khi2=0.0, dealias=1)
+    lay = _lib.Layout()
+    assert lib.psx_query_layout(ctypes.byref(p), ctypes.byref(lay)) == 0, lib.psx_last_error()
+    m = grid.dealias_mask(n, 2, "phase-shift")
+    assert lay.jm == int(np.nonzero(m.any(axis=0))[0].max()) + 1
+    assert lay.jm <= n // 2 and lay.ldw >= lay.jm
+    p.dealias = 2
+    assert lib.psx_query_layout(ctypes.byref(p), ctypes.byref(lay)) != 0

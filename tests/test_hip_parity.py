@@ -231,3 +231,93 @@ def test_config2_8192_matches_oracle(k0):
     assert err < 1e-5, err
     assert abs(e_gpu - e_ref) < 1e-6 * e_ref
     assert abs(z_gpu - z_ref) < 1e-6 * z_ref
+
+
+# ----------------------------------------------------------------- phase-shift dealiasing
+@pytest.mark.parametrize("name", VARIANTS)
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+@pytest.mark.parametrize("dim,form", [(2, "basdevant"), (3, "basdevant"), (3, "rotational")])
+def test_phase_shift_matches_oracle(name, dtype, dim, form):
+    """Phase-shift dealiasing (numpy_ref.py:68-72,125-153): the spherical mask and, per step,
+    a random grid shift from the oracle's own generator and seed, shifted by pi/n for the last
+    two stages.  Same protocol and bounds as the 2/3-rule parity cases."""
+    n = 32 if dim == 2 else 16
+    print(f"case phase-shift {name} {dtype} {dim}D {form} n={n}", flush=True)
+    cfg = Config(n=n, dim=dim, nu=0.01, alpha=0.5, force_amp=0.3, force_klo=1, force_khi=2.5,
+                 dtype=dtype, form=form, dealias="phase-shift")
+    s = _make(name, cfg)
+    ref = O.OracleSolver(cfg.with_(dtype="float64"))
+    ic = _ic(n, dim)
+    s.set_state(ic)
+    ref.set_state(ic)
+    s.advance(5, 0.01)
+    ref.advance(5, 0.01)
+    s.set_forcing_amplitude(0.1)
+    ref.set_forcing_amplitude(0.1)
+    s.advance(3, 0.007)
+    ref.advance(3, 0.007)
+    err = O.rel_err(s.get_state(), ref.get_state())
+    print(f"  rel err {err:.3e}", flush=True)
+    assert err < TOL[dtype], err
+    # the retained set is the disc / ball, wider than the 2/3 rule's square / cube
+    got = s.get_state()
+    mask = O.dealias_mask(n, dim, "phase-shift")
+    assert np.all(got[..., ~mask] == 0)
+    assert np.abs(got[..., mask & ~O.dealias_mask(n, dim)]).max() > 0
+
+
+def test_phase_shift_golden_trajectory():
+    """The reference's own phase-shift run (tests/golden/trajectories.npz, out_3d_phase: 3D
+    16^3 fp64, nu = 0.02, 3 steps of 0.01, seed 1234)."""
+    from pathlib import Path
+
+    g = np.load(Path(__file__).resolve().parent / "golden" / "trajectories.npz")
+    cfg = Config(n=16, dim=3, nu=0.02, dtype="float64", dealias="phase-shift")
+    s = _make("hip/fused", cfg)
+    s.set_state(g["ic_3d_phase"])
+    s.advance(3, 0.01)
+    err = O.rel_err(s.get_state(), g["out_3d_phase"])
+    print(f"  golden phase-shift rel err {err:.3e}", flush=True)
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("n,dim,dtype", [(96, 2, "float64"), (256, 2, "float32"),
+                                         (1024, 2, "float32"), (48, 3, "float32"),
+                                         (64, 3, "float64")])
+def test_phase_shift_sizes(n, dim, dtype):
+    """Larger grids on the run-time kernels: radix-3 lengths, the four-step split (1024^2) and
+    3D; eager and graph-replayed steps agree with each other and with the oracle."""
+    print(f"case phase-shift {dim}D n={n} {dtype}", flush=True)
+    cfg = Config(n=n, dim=dim, nu=1e-3 if dim == 2 else 1 / 1600, dtype=dtype,
+                 dealias="phase-shift")
+    a, b = _make("hip/fused", cfg), _make("hip/fused-graph", cfg)
+    ref = O.OracleSolver(cfg.with_(dtype="float64"))
+    if dim == 2:
+        ic = initial.random_phase_2d(n, k0=min(10.0, n / 12), seed=3)
+        dt = 0.2 * (2 * math.pi / n)
+    else:
+        ic = initial.to_spectral(initial.taylor_green_3d(n), 3) + 0.01 * _ic(n, 3, k0=3, seed=4)
+        dt = 0.02
+    for s in (a, b, ref):
+        s.set_state(ic)
+        s.advance(2, dt)
+    assert np.array_equal(a.get_state(), b.get_state())
+    err = O.rel_err(a.get_state(), ref.get_state())
+    print(f"  rel err {err:.3e}", flush=True)
+    assert err < TOL[dtype], err
+
+
+def test_phase_shift_has_no_device_loop():
+    from pyspectral_amd import simulate
+
+    cfg = Config(n=32, dim=2, nu=0.01, dtype="float64", dealias="phase-shift")
+    s = _make("hip/fused", cfg)
+    s.set_state(_ic(32, 2))
+    assert not s.device_loop
+    with pytest.raises(BackendUnavailable):
+        s.run_device(0.1, dt=0.01)
+    ref = O.OracleSolver(cfg)
+    ref.set_state(_ic(32, 2))
+    res = simulate.run(s, 0.05, dt=0.01)  # the host loop
+    ref.advance(5, 0.01)
+    assert res.steps == 5 and O.rel_err(s.get_state(), ref.get_state()) < 1e-12
