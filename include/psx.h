@@ -61,7 +61,7 @@ typedef struct psx_layout {
 /* Plan the kernels for p and report the buffer sizes.  Replaces the buffer and plan set-up
  * of src/pyspectral/backends/cupy_cuda.py:40-82,100-112 (CupyCudaSolver.__init__,
  * _make_plans).  Returns non-zero for unsupported sizes (n must be even, >= 8, of the
- * form 2^a or 3*2^a). */
+ * form 2^a, 3*2^a or 5*2^a with a >= 2). */
 int psx_query_layout(const psx_flow_params* p, psx_layout* out);
 
 /* Create a context on the current device borrowing u0, acc and work.

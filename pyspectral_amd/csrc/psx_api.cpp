@@ -69,6 +69,7 @@ int pow2_floor(int x) {
 
 // Elements per thread for a transform kernel over length L in the given precision.
 int pick_E(int L, bool fp64, bool heavy) {
+  if (L % 5 == 0) return 20;  // lengths 5 * 2^a: one radix-5 pass
   const bool r3 = (L % 3 == 0);
   if (r3) return (fp64 || heavy) ? 12 : 24;
   if (fp64) return heavy ? 8 : 16;
@@ -167,7 +168,7 @@ namespace {
 bool supported_n(int n) {
   if (n < 8 || n % 2) return false;
   Factors f = factor(n);
-  return f.ok && f.b5 == 0 && f.b3 <= 1 && f.a >= 2;
+  return f.ok && f.b3 + f.b5 <= 1 && f.a >= 2;
 }
 
 // Fill the flow geometry and launch shapes (no device allocation).
@@ -175,7 +176,8 @@ template <typename T>
 int plan_flow(const psx_flow_params& p, psx_ctx* c, Flow<T>* f) {
   const bool fp64 = sizeof(T) == 8;
   const int n = p.n;
-  if (!supported_n(n)) return fail("psx: n=%d unsupported (even, >= 8, 2^a or 3*2^a)", n);
+  if (!supported_n(n))
+    return fail("psx: n=%d unsupported (even, >= 8, of the form 2^a, 3*2^a or 5*2^a, a >= 2)", n);
   if (p.dim != 2 && p.dim != 3) return fail("psx: dim must be 2 or 3");
   if (p.dim == 2 && p.form != 0) return fail("psx: 2D supports the basdevant form only");
   if (p.form != 0 && p.form != 1) return fail("psx: form must be 0 or 1");

@@ -105,6 +105,7 @@ struct Geo2D {
 };
 
 constexpr int ct_pick_E(int L, bool fp64) {
+  if (L % 5 == 0) return 20;
   if (L % 3 == 0) return fp64 ? 12 : 24;
   return 16;
 }

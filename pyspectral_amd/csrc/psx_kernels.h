@@ -26,13 +26,15 @@ inline bool set_lds_max(const void* fn) {
 }
 
 // Elements-per-thread variants compiled for the transform kernels (the planner in psx_api.cpp
-// only picks these): 2D fp32 {8,16,24,32}, fp64 {8,12,16}; 3D {8,12,16}.
+// only picks these): 2D fp32 {8,16,20,24,32}, fp64 {8,12,16,20}; 3D {8,12,16,20}.  20 serves
+// the lengths 5 * 2^a (radix-5 pass).
 template <typename T>
 constexpr bool e_ok_2d(int E) {
-  return sizeof(T) == 4 ? (E == 8 || E == 16 || E == 24 || E == 32) : (E == 8 || E == 12 || E == 16);
+  return sizeof(T) == 4 ? (E == 8 || E == 16 || E == 20 || E == 24 || E == 32)
+                        : (E == 8 || E == 12 || E == 16 || E == 20);
 }
-constexpr bool e_ok_3d(int E) { return E == 8 || E == 12 || E == 16; }
-#define PSX_E_CASES PSX_CASE(8) PSX_CASE(12) PSX_CASE(16) PSX_CASE(24) PSX_CASE(32)
+constexpr bool e_ok_3d(int E) { return E == 8 || E == 12 || E == 16 || E == 20; }
+#define PSX_E_CASES PSX_CASE(8) PSX_CASE(12) PSX_CASE(16) PSX_CASE(20) PSX_CASE(24) PSX_CASE(32)
 
 template <typename T>
 hipError_t launch_line2d(const Flow<T>& f, const Launch& lc, int flags, hipStream_t st);

@@ -39,8 +39,8 @@ def test_library_exports_every_declared_symbol():
 @pytest.mark.skipif(not _lib.LIB_PATH.exists(), reason="libpsx.so not built")
 @pytest.mark.parametrize("n,dim,form,fp64,ok", [
     (8192, 2, 0, 0, True), (256, 2, 0, 1, True), (32, 2, 0, 0, True), (16, 3, 1, 1, True),
-    (384, 3, 0, 0, True), (96, 2, 0, 0, True), (10, 2, 0, 0, False), (32, 2, 1, 0, False),
-    (7, 2, 0, 0, False)])
+    (384, 3, 0, 0, True), (96, 2, 0, 0, True), (20, 2, 0, 1, True), (40, 3, 0, 0, True),
+    (10, 2, 0, 0, False), (60, 2, 0, 0, False), (32, 2, 1, 0, False), (7, 2, 0, 0, False)])
 def test_layout_query(n, dim, form, fp64, ok):
     lib = _lib.load()
     p = _lib.FlowParams(n=n, dim=dim, form=form, fp64=fp64, nu=1e-3, alpha=0.0, ka2=1e30,
@@ -61,8 +61,8 @@ def test_layout_query(n, dim, form, fp64, ok):
 
 
 @pytest.mark.skipif(not _lib.LIB_PATH.exists(), reason="libpsx.so not built")
-@pytest.mark.parametrize("n", [8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768,
-                               1024, 1536, 2048, 4096, 8192])
+@pytest.mark.parametrize("n", [8, 12, 16, 20, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256,
+                               320, 384, 512, 640, 768, 1024, 1280, 1536, 2048, 4096, 8192])
 @pytest.mark.parametrize("dim,form", [(2, 0), (3, 0), (3, 1)])
 @pytest.mark.parametrize("fp64", [0, 1])
 def test_planner_picks_compiled_variants(n, dim, form, fp64):
@@ -75,10 +75,11 @@ def test_planner_picks_compiled_variants(n, dim, form, fp64):
                         klo2=0.0, khi2=0.0)
     lay = _lib.Layout()
     assert lib.psx_query_layout(ctypes.byref(p), ctypes.byref(lay)) == 0, lib.psx_last_error()
-    ok2d = {8, 16, 24, 32} if not fp64 else {8, 12, 16}
+    ok2d = {8, 16, 20, 24, 32} if not fp64 else {8, 12, 16, 20}
     # 3D fp32 sizes with compile-time kernels: any plan the planner makes is compiled (24 for
     # radix-3 lines, 4 for the narrow column tiles of 64^3)
-    ok3d = {4, 8, 12, 16, 24} if (not fp64 and n in (64, 96, 128, 192, 256, 384)) else {8, 12, 16}
+    ok3d = ({4, 8, 12, 16, 24} if (not fp64 and n in (64, 96, 128, 192, 256, 384))
+            else {8, 12, 16, 20})
     ok = ok2d if dim == 2 else ok3d
     for e, t in ((lay.e_line, lay.t_line), (lay.e_spec, lay.t_spec),
                  (lay.e_outer, lay.t_outer), (lay.e_mid, lay.t_mid)):

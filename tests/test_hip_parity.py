@@ -78,7 +78,9 @@ def test_tgv2d_exact(name):
                                            (256, "float64", 2), (384, "float32", 2),
                                            (512, "float32", 2), (1024, "float32", 2),
                                            (1024, "float64", 2), (2048, "float32", 1),
-                                           (4096, "float32", 1)])
+                                           (4096, "float32", 1), (20, "float64", 3),
+                                           (40, "float32", 3), (160, "float64", 2),
+                                           (320, "float32", 2), (1280, "float32", 1)])
 def test_2d_sizes_and_four_step(n, dtype, steps):
     print(f"case 2D n={n} {dtype}", flush=True)
     cfg = Config(n=n, dim=2, nu=1e-3, alpha=0.02, force_amp=0.2, force_klo=2, force_khi=4,
@@ -106,7 +108,10 @@ def test_2d_sizes_and_four_step(n, dtype, steps):
                                           (32, "basdevant", "float64"),
                                           (256, "basdevant", "float32"),
                                           (384, "basdevant", "float32"),
-                                          (384, "rotational", "float32")])
+                                          (384, "rotational", "float32"),
+                                          (20, "rotational", "float64"),
+                                          (40, "basdevant", "float32"),
+                                          (80, "basdevant", "float32")])
 def test_3d_sizes(n, form, dtype):
     print(f"case 3D n={n} {form} {dtype}", flush=True)
     cfg = Config(n=n, dim=3, nu=1 / 1600, dtype=dtype, form=form)

@@ -7,7 +7,7 @@
 - IF-RK4 temporal self-convergence on a smooth nonlinear flow (validation.py:86-100): order
   near 4 and the published error at every dt (results/validation_2d.json, "temporal").
 - Spectral convergence in n (validation.py:103-119) against the published errors at the grid
-  sizes the kernels support (2^a and 3 * 2^a; results/validation_2d.json, "spatial").
+  sizes of the protocol (8 ... 64, 20 and 40 included; results/validation_2d.json, "spatial").
 
 Maximum relative errors are max|a - b| / max|b| (validation.py:36-38).
 """
@@ -33,8 +33,8 @@ PUB_TEMPORAL = {0.04: 4.3819985044200083e-07, 0.02: 2.777248871034911e-08,
                 0.0025: 6.8145811136031484e-12}
 PUB_TEMPORAL_ORDER = 3.993782286607557
 PUB_SPATIAL = {8: 0.09457572679027483, 12: 0.06184973471137069, 16: 0.005421877805026815,
-               24: 0.0006398754099795953, 32: 7.81320123782286e-06, 48: 7.803592630563717e-08,
-               64: 6.137047484046012e-11}
+               20: 0.000707740167908263, 24: 0.0006398754099795953, 32: 7.81320123782286e-06,
+               40: 7.900165112528533e-07, 48: 7.803592630563717e-08, 64: 6.137047484046012e-11}
 
 
 def _make(cfg, name="hip/fused"):
