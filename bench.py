@@ -17,7 +17,7 @@ fp32 and fp64) timed on a bounded sample on this host; and, unless --no-extras, 
 Taylor-Green vortex on 384^3 and the forced ensemble timed the same way (``extras``).
 
     python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-extras]
-                    [--workload single|tgv3d|ensemble|fft-yardstick]
+                    [--workload single|tgv3d|ensemble|fft-yardstick|les-conv]
 """
 
 from __future__ import annotations
@@ -217,13 +217,14 @@ def _time_events(torch, fn: Callable[[], None], repeats: int, inner: int = 1) ->
 
 
 def fft_ceiling(torch, n: int, dim: int, form: str, fp64: bool = False,
-                repeats: int = 10) -> dict[str, Any]:
+                repeats: int = 10, flows: int = 1) -> dict[str, Any]:
     """Standalone rocFFT time of one RK stage's batched transforms, the reference's
     fft_ceiling (benchmark.py:145-214) on ROCm: in-place unnormalised hipFFT plans on the
     padded layout (complex n^(d-1) x (n/2+1) <-> real n^(d-1) x (n+2)), eagerly and replayed
     from a graph of 10 stages, and torch.fft (out of place, default normalisation); the
-    ceiling is the fastest leg."""
+    ceiling is the fastest leg.  ``flows`` > 1 batches the transforms of that many flows."""
     ni, nf = FFT_SPLIT[(dim, form)]
+    ni, nf = ni * flows, nf * flows
     nw = max(ni, nf)
     nh = n // 2 + 1
     fsc = n ** (dim - 1) * nh
@@ -663,6 +664,25 @@ def ensemble_bench(a: argparse.Namespace, cpu: bool) -> dict[str, Any]:
     model_bytes = sum(traffic_model(c.n, 2, "basdevant", "float32")["total_bytes"]
                       for c in cfgs)
     value = npts / t_step / 1e6
+    fft_roof: dict[str, Any] = {}
+    if not a.no_roofline:
+        # the FFT roofline of the whole ensemble: per member size, rocFFT on the batched
+        # transforms of all members of that size, plus the reference model's pointwise bytes
+        # at the measured copy bandwidth
+        bcopy = copy_bandwidth(torch)
+        counts: dict[int, int] = {}
+        for c in cfgs:
+            counts[c.n] = counts.get(c.n, 0) + 1
+        t_fft, legs = 0.0, {}
+        for nn, cnt in sorted(counts.items()):
+            ce = fft_ceiling(torch, nn, 2, "basdevant", repeats=10, flows=cnt)
+            legs[str(nn)] = {k: v for k, v in ce.items() if k.endswith("_s") or k == "best_leg"}
+            t_fft += 4 * (ce["best_stage_s"] or math.nan)
+        t_pw = sum(traffic_model(c.n, 2, "basdevant", "float32")["pointwise_bytes"]
+                   for c in cfgs) / (bcopy * 1e9)
+        fft_roof = {"pct_fft_roofline": 100 * (t_fft + t_pw) / t_step,
+                    "roofline_step_ms": 1e3 * (t_fft + t_pw), "fft_ceiling_step_ms": 1e3 * t_fft,
+                    "copy_bandwidth_GBs": bcopy, "fft_ceiling": legs}
     out = {
         "metric": "Mpts/s per RK4 step", "value": value, "unit": "Mpts/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": t_step * 1e3,
@@ -680,7 +700,7 @@ def ensemble_bench(a: argparse.Namespace, cpu: bool) -> dict[str, Any]:
         "reference_model_effective_GBs": model_bytes / t_step / 1e9,
         "graph_ms_per_step": t_graph * 1e3, "autotune_s": tune_s,
         "autotune": tuned["timings"], "autotune_cached": tuned.get("cached", False),
-        "finite": finite,
+        "finite": finite, **fft_roof,
     }
     if cpu:
         out["cpu_baseline"] = ensemble_cpu_baseline(cfgs, states, outs)
@@ -715,6 +735,103 @@ def ensemble_cpu_baseline(cfgs, states, outs) -> dict:
                       "counts: %s s per member step" % (cores, per), "cpu": cpu_model()}
 
 
+def les_conv_bench(a: argparse.Namespace) -> dict[str, Any]:
+    """The learned closure's kernels on 64^3 (bf16 MFMA convolutions with 32 channels, their
+    input and weight gradients, the loss, Adam, a training step at batch 8), and GPU time per
+    step of the 64^3 LES (no model, Smagorinsky, the network) and of the 256^3 DNS it is
+    compared with, each on the device controller from the same kind of forced state.  Network
+    weights are random (the timing does not depend on their values)."""
+    import torch
+
+    from pyspectral_amd import _lib, initial
+    from pyspectral_amd.backends import get
+    from pyspectral_amd.config import Config
+    from pyspectral_amd.les import C, NLAYER, TAU_C, LESSolver, SgsNet, conv
+
+    lib, n = _lib.load(), 64
+    rng = np.random.default_rng(3)
+    flop_pt = 2.0 * 27 * C * C
+
+    def events(fn: Callable[[], None], reps: int) -> float:
+        # 10 calls between the events: a single call of a 20-us kernel would also time the
+        # host's launch latency
+        return float(np.median(_time_events(torch, fn, reps, inner=10)))
+
+    out: dict[str, Any] = {"unit": "us per call", "flop_per_point": flop_pt}
+    net = SgsNet(n=n, seed=1)
+    res: dict[str, Any] = {}
+    for nb in (1, 8):
+        P = nb * n ** 3
+        x = torch.from_numpy(rng.normal(size=(P, C)).astype(np.float32)).to(torch.bfloat16).cuda()
+        y = torch.zeros_like(x)
+        msk = torch.ones_like(x)
+        t_f = events(lambda: conv(x, net.wb[1], n=n, nb=nb, bias=net.b[1], y=y, relu=True), 20)
+        t_b = events(lambda: conv(x, net.wt[1], n=n, nb=nb, mask=msk, y=y), 20)
+        res[f"conv_fwd_nb{nb}"] = {"us": 1e6 * t_f, "TFLOPs": flop_pt * P / t_f / 1e12}
+        res[f"conv_input_grad_nb{nb}"] = {"us": 1e6 * t_b, "TFLOPs": flop_pt * P / t_b / 1e12}
+        if nb == 8:
+            dw = torch.zeros(27 * C * C, dtype=torch.float32, device="cuda")
+            db = torch.zeros(C, dtype=torch.float32, device="cuda")
+            st = _lib.stream_ptr()
+            for name, fn in (("wgrad_nb8", lib.psx_cnn_wgrad),
+                             ("wgrad_noalloc_nb8", lib.psx_cnn_wgrad_noalloc)):
+                t_w = events(lambda: _lib.check(fn(x.data_ptr(), y.data_ptr(), dw.data_ptr(),
+                                                   db.data_ptr(), n, nb, st)), 10)
+                res[name] = {"us": 1e6 * t_w, "TFLOPs": flop_pt * P / t_w / 1e12}
+        del x, y, msk
+    X = torch.from_numpy(rng.normal(size=(8 * n ** 3, C)).astype(np.float32)).to(
+        torch.bfloat16).cuda()
+    T = torch.from_numpy(rng.normal(size=(8 * n ** 3, TAU_C)).astype(np.float32)).cuda()
+    res["network_forward_nb1"] = {"us": 1e6 * events(lambda: net.forward(X[:n ** 3], 1), 20)}
+    loss = torch.zeros(1, dtype=torch.float64, device="cuda")
+    t_tr = events(lambda: net.train_step(X, T, 8, loss), 5)
+    # forward 6 layers, input gradients of 5, weight gradients of 6
+    res["train_step_batch8"] = {"us": 1e6 * t_tr,
+                                "TFLOPs": 17 * flop_pt * 8 * n ** 3 / t_tr / 1e12}
+    out["kernels"] = res
+    del X, T
+
+    def device_step_us(s: Any, steps: int) -> tuple[float, float]:
+        kw = dict(cfl=0.5, dt_max=0.05, cfl_every=10, chunk=10, record_start=False, depth=1,
+                  target_power=0.1)
+        s.run_device(math.inf, kmax=10, use_graph=True, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = s.run_device(math.inf, kmax=steps, use_graph=True, **kw)
+        torch.cuda.synchronize()
+        return 1e6 * (time.perf_counter() - t0) / steps, float(np.mean(r["dt"]))
+
+    nu = 1.6e-3
+    steps: dict[str, Any] = {}
+    for N, model in ((256, "dns"), (64, "none"), (64, "smagorinsky"), (64, "net")):
+        cfg = Config(n=N, dim=3, nu=nu, dtype="float32", force_klo=2.0, force_khi=3.0)
+        s = (get("hip/fused").make(cfg) if model == "dns"
+             else LESSolver(cfg, model, net=net if model == "net" else None))
+        s.set_state(initial.to_spectral(initial.random_3d(N, k0=2.5, seed=7, energy=1.0), 3))
+        us, dt = device_step_us(s, 50 if N == 256 else 200)
+        steps[f"{model}_{N}"] = {"us_per_step": us, "dt_mean": dt}
+        s.close()
+        del s
+        torch.cuda.empty_cache()
+    # GPU time per unit simulated time: each grid at the CFL step of its model-free flow (the
+    # random network's stress distorts its own flow; a trained one keeps the LES's dt)
+    dns = steps["dns_256"]
+    dns["gpu_s_per_unit_time"] = 1e-6 * dns["us_per_step"] / dns["dt_mean"]
+    dt_les = steps["none_64"]["dt_mean"]
+    for k, v in steps.items():
+        if k != "dns_256":
+            v["gpu_s_per_unit_time"] = 1e-6 * v["us_per_step"] / dt_les
+            v["dns_over_les_gpu_time"] = dns["gpu_s_per_unit_time"] / v["gpu_s_per_unit_time"]
+    out["steps"] = steps
+    out["note"] = ("GPU time per unit simulated time from one forced random state per grid "
+                   "(energy 1, band 2-3, constant power 0.1, CFL 0.5), the LES models at the "
+                   "model-free LES's mean step; the closure study (python -m "
+                   "pyspectral_amd.closure) measures it on its own flows with the trained "
+                   "network")
+    out["device"] = torch.cuda.get_device_name(0)
+    return out
+
+
 def yardstick(a: argparse.Namespace) -> dict[str, Any]:
     """Only the standalone rocFFT legs of the 8192^2 (C2R 2 + R2C 2) and 384^3 (C2R 3 + R2C 5)
     stages: the run a kernel trace of the yardstick is taken from."""
@@ -729,11 +846,11 @@ def _extra(d: dict[str, Any]) -> dict[str, Any]:
     keys = ("value", "unit", "ms_per_step", "graph_ms_per_step", "repeat_ms_per_step",
             "pct_fft_roofline", "roofline_step_ms", "rocfft_stage_ms", "roofline", "per_kernel",
             "step_hbm_GBs", "config", "finite", "vs_baseline", "autotune_s", "autotune_cached",
-            "cpu_baseline")
+            "fft_ceiling_step_ms", "cpu_baseline")
     out = {k: d[k] for k in keys if k in d}
     if "fft_ceiling" in d:
         out["fft_ceiling"] = {k: v for k, v in d["fft_ceiling"].items()
-                              if k.endswith("_s") or k == "best_leg"}
+                              if k.endswith("_s") or k == "best_leg" or isinstance(v, dict)}
     return out
 
 
@@ -751,11 +868,13 @@ def main(argv: list[str] | None = None) -> int:
     ap.add_argument("--no-extras", action="store_true",
                     help="headline only: skip the 384^3 and ensemble timings")
     ap.add_argument("--n3", type=int, default=384, help="grid of the tgv3d workload")
-    ap.add_argument("--workload", choices=("single", "ensemble", "tgv3d", "fft-yardstick"),
+    ap.add_argument("--workload", choices=("single", "ensemble", "tgv3d", "fft-yardstick",
+                                           "les-conv"),
                     default="single",
                     help="single: one 8192^2 flow (default, with the other two as extras); "
                          "ensemble: the forced ensemble; tgv3d: the 3D Taylor-Green vortex "
-                         "on n3^3; fft-yardstick: the rocFFT legs only")
+                         "on n3^3; fft-yardstick: the rocFFT legs only; les-conv: the "
+                         "closure's network kernels and the LES/DNS step times")
     a = ap.parse_args(argv)
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -765,6 +884,9 @@ def main(argv: list[str] | None = None) -> int:
         return 2
     if a.workload == "fft-yardstick":
         print(json.dumps(yardstick(a)), flush=True)
+        return 0
+    if a.workload == "les-conv":
+        print(json.dumps(les_conv_bench(a)), flush=True)
         return 0
     if a.workload == "ensemble":
         out = ensemble_bench(a, not a.no_cpu)

@@ -17,9 +17,13 @@ device; every candidate computes the same transforms, so the choice changes run 
 from __future__ import annotations
 
 import ctypes
+import hashlib
+import json
 import math
+import os
 import time
 from collections.abc import Sequence
+from pathlib import Path
 from typing import Any
 
 import numpy as np
@@ -250,14 +254,79 @@ def _time_steps(ens: EnsembleSolver, dt: float, nsteps: int, warm: int = 2) -> f
     return (time.perf_counter() - t0) / nsteps
 
 
+def _cache_path() -> Path:
+    env = os.environ.get("PSX_AUTOTUNE_CACHE")
+    return Path(env) if env else Path.home() / ".cache" / "pyspectral_amd" / "autotune.json"
+
+
+def _cache_key(cfgs: Sequence[Config], nsteps: int, grids: Sequence[int]) -> str:
+    """Device, library build and the ensemble's member sizes and counts: a tuning is reused
+    only for the same ensemble shape on the same kind of device with the same kernels."""
+    import torch
+
+    props = torch.cuda.get_device_properties(0)
+    lib = hashlib.sha256(_lib.LIB_PATH.read_bytes()).hexdigest()[:16]
+    counts: dict[int, int] = {}
+    for c in cfgs:
+        counts[c.n] = counts.get(c.n, 0) + 1
+    key = {"device": props.name, "arch": getattr(props, "gcnArchName", ""),
+           "cus": props.multi_processor_count, "lib": lib,
+           "members": sorted(counts.items()), "nsteps": nsteps, "grids": list(grids)}
+    return hashlib.sha256(json.dumps(key, sort_keys=True).encode()).hexdigest()[:24]
+
+
+def _cache_load(key: str) -> dict[str, Any] | None:
+    try:
+        return json.loads(_cache_path().read_text()).get(key)
+    except (OSError, ValueError):
+        return None
+
+
+def _cache_store(key: str, result: dict[str, Any]) -> None:
+    path = _cache_path()
+    try:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        try:
+            data = json.loads(path.read_text())
+        except (OSError, ValueError):
+            data = {}
+        data[key] = result
+        tmp = path.with_suffix(f".tmp{os.getpid()}")
+        tmp.write_text(json.dumps(data))
+        os.replace(tmp, path)
+    except OSError:
+        pass  # an unwritable cache only costs a re-tune next time
+
+
 def autotune(cfgs: Sequence[Config], *, nsteps: int = 5, grids: Sequence[int] = (1, 2, 4),
-             seed: int = 0) -> dict[str, Any]:
+             seed: int = 0, cache: bool = True) -> dict[str, Any]:
     """Choose each member size's spectral variant, then the persistent grid, by timing
     graph-replayed steps of the compiled candidates on this device.
 
     Variants are timed on a sub-ensemble holding only that size's members (same count), on a
     random state; the grid is timed on the whole ensemble with the chosen variants.  Returns
-    ``{"variants": [...] (per member), "wg_per_cu": g, "timings": {...}}``."""
+    ``{"variants": [...] (per member), "wg_per_cu": g, "timings": {...}, "cached": bool}``.
+    With ``cache`` the result is kept on disk (``$PSX_AUTOTUNE_CACHE``, default
+    ``~/.cache/pyspectral_amd/autotune.json``) under a key of the device, the library build
+    and the member sizes and counts, and reused instead of re-timing."""
+    key = _cache_key(cfgs, nsteps, grids) if cache else ""
+    if cache:
+        hit = _cache_load(key)
+        if hit is not None:
+            by_n = {int(k): v for k, v in hit["variant_by_size"].items()}
+            return {"variants": [by_n[c.n] for c in cfgs], "wg_per_cu": hit["wg_per_cu"],
+                    "timings": hit["timings"], "cached": True}
+    res = _autotune(cfgs, nsteps=nsteps, grids=grids, seed=seed)
+    if cache:
+        _cache_store(key, {"variant_by_size": {str(c.n): v for c, v in zip(cfgs,
+                                                                            res["variants"])},
+                           "wg_per_cu": res["wg_per_cu"], "timings": res["timings"]})
+    res["cached"] = False
+    return res
+
+
+def _autotune(cfgs: Sequence[Config], *, nsteps: int, grids: Sequence[int],
+              seed: int) -> dict[str, Any]:
     from pyspectral_amd import initial
 
     by_size: dict[int, list[int]] = {}

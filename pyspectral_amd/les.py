@@ -157,8 +157,10 @@ class SgsNet:
         dz, spare = g, buf[NLAYER + 1]
         for layer in range(NLAYER - 1, -1, -1):
             x_in = X if layer == 0 else buf[layer - 1]
-            _lib.check(lib.psx_cnn_wgrad(_p(x_in), _p(dz), _p(self.gw[layer]),
-                                         _p(self.gb[layer]), self.n, nb, st))
+            # the trainer runs on one stream, so the library's static partial-sum array is
+            # safe here and spares each call a stream-ordered allocation (about 50 us)
+            _lib.check(lib.psx_cnn_wgrad_noalloc(_p(x_in), _p(dz), _p(self.gw[layer]),
+                                                 _p(self.gb[layer]), self.n, nb, st))
             if layer > 0:
                 conv(dz, self.wt[layer], n=self.n, nb=nb, mask=buf[layer - 1], y=spare)
                 dz, spare = spare, dz

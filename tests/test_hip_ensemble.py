@@ -128,9 +128,17 @@ def test_device_loop_matches_single_flow_loops():
     ens.close()
 
 
-def test_autotuner_changes_time_only():
+def test_autotuner_changes_time_only(tmp_path, monkeypatch):
+    monkeypatch.setenv("PSX_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
     cfgs = [_cfg(256), _cfg(256, 1), _cfg(512, 2), _cfg(1024, 3)]
     tuned = autotune(cfgs, nsteps=2, grids=(1, 2))
+    assert not tuned["cached"]
+    # the second call reads the on-disk cache: same choices, no timing runs
+    again = autotune(cfgs, nsteps=2, grids=(1, 2))
+    assert again["cached"] and again["variants"] == tuned["variants"]
+    assert again["wg_per_cu"] == tuned["wg_per_cu"]
+    # another ensemble shape is tuned afresh
+    assert not autotune(cfgs[:2], nsteps=2, grids=(1, 2))["cached"]
     print(f"  autotune: {tuned}", flush=True)
     assert all(v in variants_for(c.n) for v, c in zip(tuned["variants"], cfgs))
     assert tuned["wg_per_cu"] in (1, 2)
