@@ -3,10 +3,15 @@
     python -m pyspectral_amd.rocprof_summary trace RUN_kernel_trace.csv [--active-us 20]
     python -m pyspectral_amd.rocprof_summary traffic FETCH_counter_collection.csv \
         WRITE_counter_collection.csv [--kernels SUBSTR,...]
+    python -m pyspectral_amd.rocprof_summary counters RUN_counter_collection.csv --ref COUNTER \
+        [--kernels SUBSTR,...]
 
 ``trace``: per kernel name, the number of dispatches, how many ran longer than --active-us
 (the device loop's chunks past the end of a run are no-op dispatches of a few us), their mean,
 min and max duration and the total time.
+
+``counters``: per kernel, the mean of every counter of a --pmc run over the dispatches the
+``--ref`` counter marks active.
 
 ``traffic``: per kernel, the mean FETCH_SIZE and WRITE_SIZE counter values (KiB per dispatch)
 over its dispatches that moved data, scaled into bytes by a calibration from the same runs: a
@@ -104,6 +109,27 @@ def traffic(fetch: Path, write: Path, kernels: list[str] | None) -> dict:
     return res
 
 
+def counters(path: Path, kernels: list[str], ref: str) -> dict[str, dict[str, float]]:
+    """Per kernel, the mean of every counter of a --pmc run over the kernel's active dispatches:
+    those whose ``ref`` counter reaches half its largest value for that kernel (the device
+    loop's no-op dispatches past the end of a run do next to nothing)."""
+    by: dict[str, dict[str, dict[str, float]]] = defaultdict(lambda: defaultdict(dict))
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            k = _short(r["Kernel_Name"], 80)
+            if kernels and not any(x in k for x in kernels):
+                continue
+            by[k][r["Dispatch_Id"]][r["Counter_Name"]] = float(r["Counter_Value"])
+    out = {}
+    for k, disp in by.items():
+        top = max(d.get(ref, 0.0) for d in disp.values())
+        act = [d for d in disp.values() if d.get(ref, 0.0) >= 0.5 * top]
+        names = sorted({c for d in act for c in d})
+        out[k] = {"dispatches": len(act),
+                  **{c: sum(d.get(c, 0.0) for d in act) / len(act) for c in names}}
+    return out
+
+
 def main(argv: list[str] | None = None) -> int:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -114,7 +140,15 @@ def main(argv: list[str] | None = None) -> int:
     p.add_argument("fetch", type=Path)
     p.add_argument("write", type=Path)
     p.add_argument("--kernels", default="")
+    q = sub.add_parser("counters")
+    q.add_argument("csv", type=Path)
+    q.add_argument("--ref", required=True, help="counter that marks a dispatch active")
+    q.add_argument("--kernels", default="")
     a = ap.parse_args(argv)
+    if a.cmd == "counters":
+        print(json.dumps(counters(a.csv, [s for s in a.kernels.split(",") if s], a.ref),
+                         indent=1))
+        return 0
     if a.cmd == "trace":
         print(trace_table(a.csv, a.active_us))
         return 0
