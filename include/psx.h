@@ -256,6 +256,9 @@ typedef struct psx_sgs_desc {
   double backscatter; /* network: factor in [0, 2] on the stress where -tau_ij S_ij < 0, i.e.
                          where it would move energy to the resolved scales (1 keeps it, 0 clips
                          it, > 1 amplifies it) */
+  const float* shell_gain; /* optional device array: factor on the source per wavenumber shell
+                              (int)(|k| + 0.5), floor(sqrt(3) L / 2 + 0.5) + 1 entries; null:
+                              none */
 } psx_sgs_desc;
 int psx_set_sgs(psx_ctx* c, const psx_sgs_desc* d);
 int psx_sgs_update(psx_ctx* c, void* stream);
@@ -285,7 +288,7 @@ typedef struct psx_les_args {
 #define PSX_LES_SMAGORINSKY 6 /* physical gradient pairs -> stress pairs, scale = (Cs D)^2 */
 #define PSX_LES_TAU_FROM_NET 7 /* in2: fp32 [L^3][32] model output -> stress pairs, times scale; with in = physical gradient pairs, the stress is multiplied by scale2 where -tau_ij S_ij < 0 (backscatter; 0 clips it), and for scale3 > 0 the Smagorinsky stress with (Cs D)^2 = scale3 is added */
 #define PSX_LES_TARGET 8     /* filtered product pairs, in2: velocity pairs -> fp32 [L^3][8] stress, times scale */
-#define PSX_LES_SOURCE 9     /* stress pair spectra -> projected -i k_j tau_ij, state layout (pitch) */
+#define PSX_LES_SOURCE 9     /* stress pair spectra -> projected -i k_j tau_ij, state layout (pitch); in2: null or per-shell factors (as psx_sgs_desc.shell_gain) */
 #define PSX_LES_SUMSQ 10     /* in2: n fp32 values in rows of `pitch`, first L2 of each -> += sum of squares (double) */
 int psx_les_op(int which, const psx_les_args* a, void* stream);
 

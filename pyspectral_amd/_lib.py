@@ -63,7 +63,8 @@ class SgsDesc(ctypes.Structure):
     _fields_ = [("model", c_int), ("src", c_void_p), ("full", c_void_p), ("grad", c_void_p),
                 ("tau", c_void_p), ("x", c_void_p), ("w", c_void_p * 6), ("b", c_void_p * 6),
                 ("act", c_void_p * 5), ("out", c_void_p), ("g_scale", c_double),
-                ("tau_scale", c_double), ("smag", c_double), ("backscatter", c_double)]
+                ("tau_scale", c_double), ("smag", c_double), ("backscatter", c_double),
+                ("shell_gain", c_void_p)]
 
 
 SGS_MODELS = {"smagorinsky": 1, "net": 2}  # PSX_SGS_*

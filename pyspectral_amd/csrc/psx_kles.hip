@@ -349,9 +349,10 @@ __global__ void k_les_target(const cx<float>* __restrict__ fp, const cx<float>* 
 }
 
 // Forward-transformed stress pairs -> S_i = -i k_j tau_ij, Leray-projected, 2/3-masked, in the
-// state layout (3 components of L rows x L rows x pitch; columns kz <= L/2).
+// state layout (3 components of L rows x L rows x pitch; columns kz <= L/2); with gain, times
+// gain[(int)(|k| + 0.5)] (the shell index of the spectrum).
 __global__ void k_les_source(const cx<float>* __restrict__ th, int L, int pitch,
-                             cx<float>* __restrict__ src) {
+                             const float* __restrict__ gain, cx<float>* __restrict__ src) {
   const int nh = L / 2 + 1;
   const long long n = (long long)L * L * nh;
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -388,6 +389,13 @@ __global__ void k_les_source(const cx<float>* __restrict__ th, int L, int pitch,
   s0 = csub(s0, cscl(fx, kd));
   s1 = csub(s1, cscl(fy, kd));
   s2 = csub(s2, cscl(fz, kd));
+  if (gain) {
+    // |k|^2 <= 3 (L/2)^2 is an integer, so sqrt(k2) is never within fp32 rounding of n + 1/2
+    const float g = gain[(int)(sqrtf((float)k2) + 0.5f)];
+    s0 = cscl(g, s0);
+    s1 = cscl(g, s1);
+    s2 = cscl(g, s2);
+  }
   src[so] = s0;
   src[cst + so] = s1;
   src[2 * cst + so] = s2;
@@ -458,7 +466,7 @@ hipError_t launch_les(int which, const LesArgs& a, hipStream_t st) {
     case kLesSource: {
       const long long nm = (long long)a.L * a.L * (a.L / 2 + 1);
       hipLaunchKernelGGL(k_les_source, grid1(nm), dim3(256), 0, st, a.cin, a.L, a.pitch,
-                         a.cout);
+                         static_cast<const float*>(a.in2), a.cout);
       break;
     }
     case kLesSumSq:

@@ -95,7 +95,7 @@ int enqueue_sgs(const psx_sgs_desc& d, const cx<float>* u0, int pitch, int L, hi
                d.backscatter, d.smag > 0.0 ? d.smag : 0.0));
   }
   PSX_HIP(launch_c2c3(tau, tw, L, 3, false, st));
-  PSX_HIP(op(kLesSource, tau, nullptr, d.src, 0, pitch, 1.0));
+  PSX_HIP(op(kLesSource, tau, d.shell_gain, d.src, 0, pitch, 1.0));
   return 0;
 }
 

@@ -26,7 +26,8 @@ enum : int {
                        //   cin (or null): physical gradient pairs -> backscatter times scale2,
                        //   plus the Smagorinsky stress for scale3 = (Cs D)^2 > 0
   kLesTarget = 8,      // cin: filtered product pairs, in2: velocity pairs -> out: fp32 [L^3][8]
-  kLesSource = 9,      // cin: stress pair spectra, pitch -> cout: source in the state layout
+  kLesSource = 9,      // cin: stress pair spectra, pitch, in2: null or per-shell factors
+                       // -> cout: source in the state layout
   kLesSumSq = 10,      // in2: fp32 values, n = count, pitch = width, L2 = channels counted
                        //   -> out: double sum of squares (accumulated)
 };

@@ -247,13 +247,15 @@ class LESSolver(HipSolver):
     would transfer energy to the resolved scales (-tau_ij S_ij < 0): 1 keeps the network's
     output as it is, 0 clips the backscatter, a factor up to 2 amplifies it.  ``mix_cs`` > 0 adds a Smagorinsky stress of that
     constant to the network's (a mixed model); ``cs`` is the constant of the pure Smagorinsky
-    model."""
+    model.  ``shell_gain`` (one factor per wavenumber shell (int)(|k| + 0.5), shells past its
+    end get 1) multiplies the model's source shell by shell."""
 
     variant = "les"
 
     def __init__(self, cfg: Config, model: str = "none", *, net: SgsNet | None = None,
                  cs: float = 0.17, kc: float = 21.0, g_scale: float = 1.0,
-                 tau_scale: float = 1.0, backscatter: float = 1.0, mix_cs: float = 0.0) -> None:
+                 tau_scale: float = 1.0, backscatter: float = 1.0, mix_cs: float = 0.0,
+                 shell_gain: Any = None) -> None:
         if cfg.dim != 3 or cfg.dtype != "float32":
             raise BackendUnavailable("LES: 3D fp32 flows")
         if model not in ("none", "smagorinsky", "net"):
@@ -271,6 +273,14 @@ class LESSolver(HipSolver):
         self.grad = torch.zeros(6 * L ** 3, dtype=torch.complex64, device="cuda")
         self.tau = torch.zeros(3 * L ** 3, dtype=torch.complex64, device="cuda")
         self.X = torch.zeros((L ** 3, C), dtype=torch.bfloat16, device="cuda")
+        self.gain = None
+        if shell_gain is not None:
+            g = np.ones(self.diag.nshell, dtype=np.float32)
+            sg = np.asarray(shell_gain, dtype=np.float32)[: self.diag.nshell]
+            if not np.all(np.isfinite(sg)):
+                raise ValueError("shell_gain: non-finite factor")
+            g[: len(sg)] = sg
+            self.gain = torch.from_numpy(g).cuda()
         if model == "none":
             return
         d = _lib.SgsDesc(model=_lib.SGS_MODELS[model], src=_p(self.src).value,
@@ -278,7 +288,7 @@ class LESSolver(HipSolver):
                          tau=_p(self.tau).value, x=_p(self.X).value,
                          g_scale=g_scale, tau_scale=tau_scale,
                          smag=((cs if model == "smagorinsky" else mix_cs) * math.pi / kc) ** 2,
-                         backscatter=backscatter)
+                         backscatter=backscatter, shell_gain=_p(self.gain).value)
         if model == "net":
             assert net is not None and net.n == L
             acts = net._buffers(1)
