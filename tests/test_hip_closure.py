@@ -4,7 +4,8 @@ against the time-averaged filtered-DNS spectrum.  The network must beat the coar
 without a model on the E(k) error up to the filter cutoff, and train to a validation loss
 well below that of predicting zero.  Before that, the a-priori energy-transfer machinery: the
 stress source of the LES kernels against the oracle's, shell by shell, and its sum against
-the physical-space mean of tau_ij du_i/dx_j; the source kernel's per-shell factor."""
+the physical-space mean of tau_ij du_i/dx_j; the source kernel's per-shell factor; and the
+per-shell factor fitted a posteriori on a window before the reported one."""
 
 from __future__ import annotations
 
@@ -84,6 +85,7 @@ def test_trained_network_beats_no_model():
     print("case closure study (short)", flush=True)
     r = closure_study(t_spin=3.0, n_snap=40, snap_every=0.1, n_val=8, train_steps=200,
                       t_les=6.0, t_avg=3.0, t_cost=1.0, backscatter=(1.0,), mixed=((1.0, 0.04),),
+                      fit_models=("net_bs1",), fit_iters=2, fit_t_les=3.0, fit_t_avg=2.0,
                       log=lambda m: print("  " + m, flush=True))
     les = r["les"]
     assert all(les[m]["finite"] for m in les)
@@ -102,3 +104,9 @@ def test_trained_network_beats_no_model():
     held = ap["errors"]["held_out"]
     assert held["net_bs1_shellgain"]["rel_err_mean"] < held["net_bs1"]["rel_err_mean"]
     assert les["net_bs1_shellgain"]["finite"]
+    # a posteriori: the per-shell factor is fitted on [1, 3] and reported on [3, 6]; the
+    # returned iterate is the one with the smallest maximum error on the fit window
+    fit = r["aposteriori_fit"]["net_bs1"]
+    assert fit["disjoint_from_report"] and len(fit["history"]) == 3
+    assert fit["err_max"] == min(h["err_max"] for h in fit["history"])
+    assert les["net_bs1_fit"]["finite"]
