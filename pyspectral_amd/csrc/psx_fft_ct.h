@@ -100,6 +100,80 @@ __device__ __forceinline__ void fft_ct(cx<T> (&v)[E], cx<T>* lds,
   });
 }
 
+// NS independent transforms of the same plan advanced pass by pass together: sequence s
+// exchanges through its own LDS region at lds + s * STRIDE (elements), each pass loads a
+// twiddle once for all NS sequences, and the NS exchanges of a pass share one pair of
+// barriers.  Run one after another, each transform pays its own twiddle-load and barrier
+// latency; together, the loads and LDS accesses of the NS sequences are in flight at once.
+// v[0 .. NS-1] are transformed (NS <= NA).
+template <int NS, int NA, int E, int R, bool INV, int L, int Ns, int NT, class Lay, int STRIDE,
+          bool FROM, bool TO, typename T>
+__device__ __forceinline__ void ct_pass_multi(cx<T> (&v)[NA][E], cx<T>* lds,
+                                              const cx<T>* __restrict__ tw, int tid) {
+  static_assert(E % R == 0, "radix must divide the per-thread element count");
+  static_assert(NS >= 1 && NS <= NA, "sequence count");
+  constexpr int NB = E / R;
+  constexpr int Lr = L / R;
+  if constexpr (FROM) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      int j, t;
+      Lay::template decomp<Lr>(tid + b * NT, j, t);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+#pragma unroll
+        for (int m = 0; m < R; ++m) v[s][b * R + m] = lds[s * STRIDE + Lay::addr(j + m * Lr, t)];
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    int j, t;
+    Lay::template decomp<Lr>(tid + b * NT, j, t);
+    if constexpr (Ns > 1) {
+      const int k = j % Ns;
+#pragma unroll
+      for (int m = 1; m < R; ++m) {
+        const cx<T> w = tw[(m - 1) * Ns + k];
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+          v[s][b * R + m] = INV ? cmulc(v[s][b * R + m], w) : cmul(v[s][b * R + m], w);
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) dft<R, INV>(&v[s][b * R]);
+  }
+  if constexpr (TO) {
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      int j, t;
+      Lay::template decomp<Lr>(tid + b * NT, j, t);
+      const int k = j % Ns;
+      const int base = (j - k) * R + k;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) lds[s * STRIDE + Lay::addr(base + q * Ns, t)] = v[s][b * R + q];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int NS, int E, bool INV, int NT, class Lay, CtPlan P, int STRIDE, int NA, typename T>
+__device__ __forceinline__ void fft_ct_multi(cx<T> (&v)[NA][E], cx<T>* lds,
+                                             const cx<T>* __restrict__ tw_order, int tid) {
+  static_for<0, P.np>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int R = ct_radix(P, INV, q);
+    constexpr int Ns = ct_ns(P, INV, q);
+    constexpr int off = ct_twoff(P, INV, q);
+    ct_pass_multi<NS, NA, E, R, INV, P.L, Ns, NT, Lay, STRIDE, (q > 0), (q < P.np - 1)>(
+        v, lds, tw_order + off, tid);
+  });
+}
+
 // Register slot i of a pass of radix R holds element e of sequence t.
 template <int R, int L, int NT, class Lay>
 PSX_HD void ct_elem(int i, int tid, int& e, int& t) {

@@ -104,8 +104,10 @@ __device__ __forceinline__ void body_line2d(const Flow<float>& f, int tile, int 
 
 // ----------------------------------------------------------------------- spectral kernel
 // Column tile `tile` (group tile / NJT of the outer split, columns (tile % NJT) * colT ..)
-// of flow f; specNT threads, tid in [0, specNT).
-template <Geo2D G, int stage>
+// of flow f; specNT threads, tid in [0, specNT).  MS: the two fields' transforms of one
+// direction run together (fft_ct_multi, two LDS regions of n2 * colT elements) instead of one
+// after the other in one region.
+template <Geo2D G, int stage, bool MS = false>
 __device__ __forceinline__ void body_spec2d(const Flow<float>& f, int tile, int tid,
                                             unsigned char* smem_raw) {
   using T = float;
@@ -125,7 +127,9 @@ __device__ __forceinline__ void body_spec2d(const Flow<float>& f, int tile, int 
   const cx<T>* tw_fwd = f.twp + f.pcol.twoff[0][0];
   const cx<T>* tw_rev = f.twp + f.pcol.twoff[1][0];
 
-  cx<T> va[E], vb[E];
+  cx<T> vv[2][E];
+  cx<T>(&va)[E] = vv[0];
+  cx<T>(&vb)[E] = vv[1];
   if constexpr (stage != kStagePre) {
     constexpr int R = ct_radix(G.spec, false, 0);
 #pragma unroll
@@ -174,8 +178,12 @@ __device__ __forceinline__ void body_spec2d(const Flow<float>& f, int tile, int 
     }
   }
   if constexpr (stage != kStagePre) {
-    fft_ct<E, false, NT, Lay, G.spec>(va, lds, tw_fwd, tid);
-    fft_ct<E, false, NT, Lay, G.spec>(vb, lds, tw_fwd, tid);
+    if constexpr (MS) {
+      fft_ct_multi<2, E, false, NT, Lay, G.spec, L * CT>(vv, lds, tw_fwd, tid);
+    } else {
+      fft_ct<E, false, NT, Lay, G.spec>(va, lds, tw_fwd, tid);
+      fft_ct<E, false, NT, Lay, G.spec>(vb, lds, tw_fwd, tid);
+    }
   }
   {
     const IfScalars s = load_if_scalars(f.prm, NH);
@@ -214,8 +222,12 @@ __device__ __forceinline__ void body_spec2d(const Flow<float>& f, int tile, int 
     }
   }
   constexpr int RS = ct_radix(G.spec, true, G.spec.np - 1);
-  fft_ct<E, true, NT, Lay, G.spec>(va, lds, tw_rev, tid);
-  fft_ct<E, true, NT, Lay, G.spec>(vb, lds, tw_rev, tid);
+  if constexpr (MS) {
+    fft_ct_multi<2, E, true, NT, Lay, G.spec, L * CT>(vv, lds, tw_rev, tid);
+  } else {
+    fft_ct<E, true, NT, Lay, G.spec>(va, lds, tw_rev, tid);
+    fft_ct<E, true, NT, Lay, G.spec>(vb, lds, tw_rev, tid);
+  }
 #pragma unroll
   for (int i = 0; i < E; ++i) {
     int e, t;
@@ -236,8 +248,9 @@ __device__ __forceinline__ void body_spec2d(const Flow<float>& f, int tile, int 
 
 // ------------------------------------------------------------------- outer four-step pass
 // Outer-axis transforms of column tile `tile` (inner index tile / NJT, columns
-// (tile % NJT) * colT ..) of flow f; outNT threads, tid in [0, outNT).
-template <Geo2D G, bool INV>
+// (tile % NJT) * colT ..) of flow f; outNT threads, tid in [0, outNT).  MS as in
+// body_spec2d (two LDS regions of n1 * colT elements).
+template <Geo2D G, bool INV, bool MS = false>
 __device__ __forceinline__ void body_outer2d(const Flow<float>& f, int tile, int tid,
                                              unsigned char* smem_raw) {
   using T = float;
@@ -253,7 +266,9 @@ __device__ __forceinline__ void body_outer2d(const Flow<float>& f, int tile, int
     const cx<T>* tw_o = f.twp + f.pa.twoff[INV ? 1 : 0][0];
     constexpr int R0 = ct_radix(G.outer, INV, 0);
     constexpr int RL = ct_radix(G.outer, INV, G.outer.np - 1);
-    cx<T> v0[E], v1[E];
+    cx<T> vv[2][E];
+    cx<T>(&v0)[E] = vv[0];
+    cx<T>(&v1)[E] = vv[1];
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       int e, t;
@@ -268,8 +283,12 @@ __device__ __forceinline__ void body_outer2d(const Flow<float>& f, int tile, int
         z_split(z, j0 + t == 0 ? z : zm, v0[i], v1[i]);
       }
     }
-    fft_ct<E, INV, NT, Lay, G.outer>(v0, lds, tw_o, tid);
-    fft_ct<E, INV, NT, Lay, G.outer>(v1, lds, tw_o, tid);
+    if constexpr (MS) {
+      fft_ct_multi<2, E, INV, NT, Lay, G.outer, L * CT>(vv, lds, tw_o, tid);
+    } else {
+      fft_ct<E, INV, NT, Lay, G.outer>(v0, lds, tw_o, tid);
+      fft_ct<E, INV, NT, Lay, G.outer>(v1, lds, tw_o, tid);
+    }
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       int e, t;

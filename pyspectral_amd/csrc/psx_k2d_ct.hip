@@ -12,6 +12,15 @@ struct G2 {
   static_assert(g.ok, "no 2D plan for this size");
 };
 
+// The spectral kernel of the four-step sizes and the inverse outer pass transform both fields
+// of a direction together (two LDS regions; body_spec2d, body_outer2d): 8192^2 spectral
+// 278 -> 245 us, inverse outer 158 -> 148 us.  The forward outer pass keeps one region: its
+// 64-thread work-groups at 16 KB each are held below four waves per SIMD by the LDS
+// (154 -> 161 us with two).  One-pass sizes (whole columns of up to 64 KB per work-group) keep
+// one region: two would halve their work-groups per CU.
+template <int N>
+constexpr bool kMs2 = G2<N>::g.n1 > 1;
+
 template <int N>
 __global__ void __launch_bounds__(G2<N>::g.lineNT, 1) kc_line2d(Flow<float> f, int flags) {
   if (flow_inactive(f.prm, f.nh)) return;
@@ -23,7 +32,7 @@ template <int N, int stage>
 __global__ void __launch_bounds__(G2<N>::g.specNT, 2) kc_spec2d(Flow<float> f) {
   if (flow_inactive(f.prm, f.nh)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  body_spec2d<G2<N>::g, stage>(f, blockIdx.x, threadIdx.x, smem_raw);
+  body_spec2d<G2<N>::g, stage, kMs2<N>>(f, blockIdx.x, threadIdx.x, smem_raw);
 }
 
 template <int N, bool INV>
@@ -31,7 +40,7 @@ __global__ void __launch_bounds__(G2<N>::g.outNT > 0 ? G2<N>::g.outNT : 64, 1)
     kc_outer2d(Flow<float> f) {
   if (flow_inactive(f.prm, f.nh)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  body_outer2d<G2<N>::g, INV>(f, blockIdx.x, threadIdx.x, smem_raw);
+  body_outer2d<G2<N>::g, INV, INV>(f, blockIdx.x, threadIdx.x, smem_raw);
 }
 
 // ------------------------------------------------------------------------------ launchers
@@ -62,7 +71,7 @@ static hipError_t ct_launch(const Flow<float>& f, int which, int arg, hipStream_
       break;
     }
     case 1: {
-      const size_t lds = (size_t)G.n2 * G.colT * 8;
+      const size_t lds = (size_t)(kMs2<N> ? 2 : 1) * G.n2 * G.colT * 8;
       const dim3 grid(G.n1 * NJT), block(G.specNT);
       switch (arg) {
         case kStagePre: hipLaunchKernelGGL((kc_spec2d<N, kStagePre>), grid, block, lds, st, f); break;
@@ -77,7 +86,7 @@ static hipError_t ct_launch(const Flow<float>& f, int which, int arg, hipStream_
     case 2:
     case 3: {
       if constexpr (G.n1 > 1) {
-        const size_t lds = (size_t)G.n1 * G.colT * 8;
+        const size_t lds = (size_t)(which == 3 ? 2 : 1) * G.n1 * G.colT * 8;
         if (which == 2) {
           auto fn = kc_outer2d<N, false>;
           hipLaunchKernelGGL(fn, dim3(G.n2 * NJT), dim3(G.outNT), lds, st, f);

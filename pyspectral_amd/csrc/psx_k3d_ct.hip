@@ -115,53 +115,50 @@ __global__ void __launch_bounds__(G3<N>::g.lineNT, 1) kc_line3d(Flow<float> f, i
     }
   };
 
-  cx<T> z1[E], z2[E];
-  load_pair(z1, 0, 1);
-  fft_ct<E, true, NT, Lay, G.line>(z1, lds, tw_rev, tid);
+  // The transforms of one direction run together (fft_ct_multi): their twiddle loads, LDS
+  // exchanges and barriers are shared pass by pass, one LDS region of ST elements each.
+  constexpr int ST = RT * Lay::LP;
+  cx<T> z[3][E];
+  load_pair(z[0], 0, 1);
   if constexpr (!ROT) {
-    load_single(z2, 2);
-    fft_ct<E, true, NT, Lay, G.line>(z2, lds, tw_rev, tid);
+    load_single(z[1], 2);
+    fft_ct_multi<2, E, true, NT, Lay, G.line, ST>(z, lds, tw_rev, tid);
     if (flags & kLineWantMax) {
       float m[3] = {0.f, 0.f, 0.f};  // |u_i| maxima (exact in fp32)
 #pragma unroll
       for (int i = 0; i < E; ++i) {
-        m[0] = fmaxf(m[0], abs_or_inf(z1[i].x));
-        m[1] = fmaxf(m[1], abs_or_inf(z1[i].y));
-        m[2] = fmaxf(m[2], abs_or_inf(z2[i].x));
+        m[0] = fmaxf(m[0], abs_or_inf(z[0][i].x));
+        m[1] = fmaxf(m[1], abs_or_inf(z[0][i].y));
+        m[2] = fmaxf(m[2], abs_or_inf(z[1][i].x));
       }
       double md[3] = {(double)m[0], (double)m[1], (double)m[2]};
       block_max_to<3>(md, f.stats, reinterpret_cast<double*>(smem_raw));
     }
     if (flags & kLineMaxOnly) return;
-    cx<T> z3[E];
     const T third = (T)1 / (T)3;
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      const T u = z1[i].x, v = z1[i].y, w = z2[i].x;
+      const T u = z[0][i].x, v = z[0][i].y, w = z[1][i].x;
       const T q = (u * u + v * v + w * w) * third;
-      z1[i] = mkc(u * u - q, v * v - q);
-      z2[i] = mkc(u * v, u * w);
-      z3[i] = mkc(v * w, (T)0);
+      z[0][i] = mkc(u * u - q, v * v - q);
+      z[1][i] = mkc(u * v, u * w);
+      z[2][i] = mkc(v * w, (T)0);
     }
-    fft_ct<E, false, NT, Lay, G.line>(z1, lds, tw_fwd, tid);
-    store_pair(z1, 0, 1);
-    fft_ct<E, false, NT, Lay, G.line>(z2, lds, tw_fwd, tid);
-    store_pair(z2, 2, 3);
-    fft_ct<E, false, NT, Lay, G.line>(z3, lds, tw_fwd, tid);
-    store_single(z3, 4);
+    fft_ct_multi<3, E, false, NT, Lay, G.line, ST>(z, lds, tw_fwd, tid);
+    store_pair(z[0], 0, 1);
+    store_pair(z[1], 2, 3);
+    store_single(z[2], 4);
   } else {
-    cx<T> z3[E];
-    load_pair(z2, 2, 3);
-    fft_ct<E, true, NT, Lay, G.line>(z2, lds, tw_rev, tid);
-    load_pair(z3, 4, 5);
-    fft_ct<E, true, NT, Lay, G.line>(z3, lds, tw_rev, tid);
+    load_pair(z[1], 2, 3);
+    load_pair(z[2], 4, 5);
+    fft_ct_multi<3, E, true, NT, Lay, G.line, ST>(z, lds, tw_rev, tid);
     if (flags & kLineWantMax) {
       float m[3] = {0.f, 0.f, 0.f};  // |u_i| maxima (exact in fp32)
 #pragma unroll
       for (int i = 0; i < E; ++i) {
-        m[0] = fmaxf(m[0], abs_or_inf(z1[i].x));
-        m[1] = fmaxf(m[1], abs_or_inf(z1[i].y));
-        m[2] = fmaxf(m[2], abs_or_inf(z2[i].x));
+        m[0] = fmaxf(m[0], abs_or_inf(z[0][i].x));
+        m[1] = fmaxf(m[1], abs_or_inf(z[0][i].y));
+        m[2] = fmaxf(m[2], abs_or_inf(z[1][i].x));
       }
       double md[3] = {(double)m[0], (double)m[1], (double)m[2]};
       block_max_to<3>(md, f.stats, reinterpret_cast<double*>(smem_raw));
@@ -169,15 +166,14 @@ __global__ void __launch_bounds__(G3<N>::g.lineNT, 1) kc_line3d(Flow<float> f, i
     if (flags & kLineMaxOnly) return;
 #pragma unroll
     for (int i = 0; i < E; ++i) {
-      const T u = z1[i].x, v = z1[i].y, w = z2[i].x;
-      const T ox = z2[i].y, oy = z3[i].x, oz = z3[i].y;
-      z1[i] = mkc(v * oz - w * oy, w * ox - u * oz);
-      z2[i] = mkc(u * oy - v * ox, (T)0);
+      const T u = z[0][i].x, v = z[0][i].y, w = z[1][i].x;
+      const T ox = z[1][i].y, oy = z[2][i].x, oz = z[2][i].y;
+      z[0][i] = mkc(v * oz - w * oy, w * ox - u * oz);
+      z[1][i] = mkc(u * oy - v * ox, (T)0);
     }
-    fft_ct<E, false, NT, Lay, G.line>(z1, lds, tw_fwd, tid);
-    store_pair(z1, 0, 1);
-    fft_ct<E, false, NT, Lay, G.line>(z2, lds, tw_fwd, tid);
-    store_single(z2, 2);
+    fft_ct_multi<2, E, false, NT, Lay, G.line, ST>(z, lds, tw_fwd, tid);
+    store_pair(z[0], 0, 1);
+    store_single(z[1], 2);
   }
 }
 
@@ -221,7 +217,9 @@ __global__ void __launch_bounds__(G3<N>::g.yNT, 1) kc_ypass3d(Flow<float> f) {
 
   static_for<0, NPAIR>([&](auto pc) {
     constexpr int fa = 2 * decltype(pc)::value, fb = fa + 1;
-    cx<T> va[E], vb[E];
+    cx<T> vv[2][E];
+    cx<T>(&va)[E] = vv[0];
+    cx<T>(&vb)[E] = vv[1];
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       int e, t;
@@ -236,8 +234,8 @@ __global__ void __launch_bounds__(G3<N>::g.yNT, 1) kc_ypass3d(Flow<float> f) {
         z_split3(z, j0 + t == 0 ? z : zm, va[i], vb[i]);
       }
     }
-    fft_ct<E, INV, NT, Lay, G.yp>(va, lds, tw_o, tid);
-    fft_ct<E, INV, NT, Lay, G.yp>(vb, lds, tw_o, tid);
+    // the pair's two transforms together (two LDS regions)
+    fft_ct_multi<2, E, INV, NT, Lay, G.yp, L * CT>(vv, lds, tw_o, tid);
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       int e, t;
@@ -304,7 +302,7 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
   // transform buffer): in registers beside the other two, the transform and the state it
   // spilled to scratch.
   constexpr bool kA2Lds = kc_spec3d_a2_lds(N);
-  cx<T>* a2l = lds + L * CT;
+  cx<T>* a2l = lds + 2 * L * CT;
 
   // state offsets of this thread's modes (out of range outside the 2/3 mask)
   unsigned soff[E];
@@ -335,16 +333,17 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
     constexpr int nf = ROT ? 3 : 5;
 #pragma unroll
     for (int i = 0; i < E; ++i) a0[i] = a1[i] = A2(i) = czero<T>();
-    static_for<0, nf>([&](auto cc) {
-      constexpr int c = decltype(cc)::value;
-      cx<T> v[E];
+    // field c's x transform, added into the accumulated components
+    auto load_x = [&](int c, cx<T>(&v)[E]) {
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         int e, t;
         ct_elem<R0, L, NT, Lay>(i, tid, e, t);
         v[i] = load_w<N>(f.W + c * FS + ((long long)e * N + iy) * LDW + j0 + t);
       }
-      fft_ct<E, false, NT, Lay, G.xp>(v, lds, tw_fwd, tid);
+    };
+    auto accum = [&](auto cc, const cx<T>(&v)[E]) {
+      constexpr int c = decltype(cc)::value;
 #pragma unroll
       for (int i = 0; i < E; ++i) {
         if constexpr (ROT) {
@@ -374,7 +373,23 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
           }
         }
       }
+    };
+    // the fields' transforms two at a time (fft_ct_multi), the odd one alone
+    static_for<0, nf / 2>([&](auto pc) {
+      constexpr int c0 = 2 * decltype(pc)::value;
+      cx<T> vv[2][E];
+      load_x(c0, vv[0]);
+      load_x(c0 + 1, vv[1]);
+      fft_ct_multi<2, E, false, NT, Lay, G.xp, L * CT>(vv, lds, tw_fwd, tid);
+      accum(std::integral_constant<int, c0>{}, vv[0]);
+      accum(std::integral_constant<int, c0 + 1>{}, vv[1]);
     });
+    if constexpr (nf % 2) {
+      cx<T> v[E];
+      load_x(nf - 1, v);
+      fft_ct<E, false, NT, Lay, G.xp>(v, lds, tw_fwd, tid);
+      accum(std::integral_constant<int, nf - 1>{}, v);
+    }
   }
   {
     const IfScalars s = load_if_scalars(f.prm, NH);
@@ -442,9 +457,9 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
     }
   }
   constexpr int nin = ROT ? 6 : 3;
-  static_for<0, nin>([&](auto cc) {
+  // next inputs: the velocity components (and, rotational, omega = i k x u)
+  auto make_in = [&](auto cc, cx<T>(&v)[E]) {
     constexpr int c = decltype(cc)::value;
-    cx<T> v[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       if constexpr (c == 0) {
@@ -463,14 +478,30 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
         else v[i] = mul_i(csub(cscl(fx, a1[i]), cscl(fy, a0[i])));
       }
     }
-    fft_ct<E, true, NT, Lay, G.xp>(v, lds, tw_rev, tid);
+  };
+  auto store_x = [&](int c, const cx<T>(&v)[E]) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       int e, t;
       ct_elem<RS, L, NT, Lay>(i, tid, e, t);
       f.W[c * FS + ((long long)e * N + iy) * LDW + j0 + t] = v[i];
     }
+  };
+  static_for<0, nin / 2>([&](auto pc) {
+    constexpr int c0 = 2 * decltype(pc)::value;
+    cx<T> vv[2][E];
+    make_in(std::integral_constant<int, c0>{}, vv[0]);
+    make_in(std::integral_constant<int, c0 + 1>{}, vv[1]);
+    fft_ct_multi<2, E, true, NT, Lay, G.xp, L * CT>(vv, lds, tw_rev, tid);
+    store_x(c0, vv[0]);
+    store_x(c0 + 1, vv[1]);
   });
+  if constexpr (nin % 2) {
+    cx<T> v[E];
+    make_in(std::integral_constant<int, nin - 1>{}, v);
+    fft_ct<E, true, NT, Lay, G.xp>(v, lds, tw_rev, tid);
+    store_x(nin - 1, v);
+  }
 }
 
 // ------------------------------------------------------------------------------ launchers
@@ -494,7 +525,8 @@ static hipError_t ct3_launch(const Flow<float>& f, int which, int arg, hipStream
       auto fn = kc_line3d<N, ROT>;
       static const bool attr_ = set_lds_max((const void*)fn);
       (void)attr_;
-      size_t lds = (size_t)G.lineT * ct_rows_pitch(N) * 8;
+      // three sequence regions (the transforms of one direction run together)
+      size_t lds = (size_t)3 * G.lineT * RowsCT<N, G.lineT>::LP * 8;
       const size_t red = (size_t)3 * G.lineNT * sizeof(double);
       if (lds < red) lds = red;
       hipLaunchKernelGGL(fn, dim3(N * N / G.lineT), dim3(G.lineNT), lds, st, f, arg);
@@ -502,7 +534,7 @@ static hipError_t ct3_launch(const Flow<float>& f, int which, int arg, hipStream
     }
     case 1:
     case 3: {
-      const size_t lds = (size_t)N * G.T * 8;
+      const size_t lds = (size_t)2 * N * G.T * 8;  // two sequence regions
       if (which == 1)
         hipLaunchKernelGGL((kc_ypass3d<N, ROT, false>), dim3(N * NJT), dim3(G.yNT), lds, st, f);
       else
@@ -510,8 +542,8 @@ static hipError_t ct3_launch(const Flow<float>& f, int which, int arg, hipStream
       break;
     }
     case 2: {
-      // transform buffer, plus the third component's slots where it lives in LDS
-      const size_t lds = (size_t)N * G.T * 8 +
+      // two transform regions, plus the third component's slots where it lives in LDS
+      const size_t lds = (size_t)2 * N * G.T * 8 +
                          (kc_spec3d_a2_lds(N) ? (size_t)G.xE * G.xNT * 8 : 0);
       const dim3 grid(N * NJT), block(G.xNT);
       static const bool attr_ = set_lds_max((const void*)kc_spec3d<N, ROT, kStagePre>) &&
