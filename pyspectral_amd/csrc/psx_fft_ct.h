@@ -22,9 +22,9 @@ struct RowsCT {
   static constexpr bool kPad32 = (L_ == 8192 && T_ == 1) || L_ == 384;
   static constexpr int L = L_, T = T_;
   static constexpr int LP = kPad32 ? L_ + (L_ >> 5) + 1 : ct_rows_pitch(L_);  // <= the latter
-  static PSX_HD int addr(int e, int t) { return t * LP + e + (e >> (kPad32 ? 5 : 4)); }
+  static constexpr PSX_HD int addr(int e, int t) { return t * LP + e + (e >> (kPad32 ? 5 : 4)); }
   template <int Lr>
-  static PSX_HD void decomp(int g, int& j, int& t) {
+  static constexpr PSX_HD void decomp(int g, int& j, int& t) {
     t = g / Lr;
     j = g - t * Lr;
   }
@@ -34,9 +34,9 @@ struct RowsCT {
 template <int T_>
 struct ColsCT {
   static constexpr int T = T_;
-  static PSX_HD int addr(int e, int t) { return e * T + t; }
+  static constexpr PSX_HD int addr(int e, int t) { return e * T + t; }
   template <int Lr>
-  static PSX_HD void decomp(int g, int& j, int& t) {
+  static constexpr PSX_HD void decomp(int g, int& j, int& t) {
     t = g % T;
     j = g / T;
   }
@@ -172,6 +172,22 @@ __device__ __forceinline__ void fft_ct_multi(cx<T> (&v)[NA][E], cx<T>* lds,
     ct_pass_multi<NS, NA, E, R, INV, P.L, Ns, NT, Lay, STRIDE, (q > 0), (q < P.np - 1)>(
         v, lds, tw_order + off, tid);
   });
+}
+
+// True when register slot i of a pass of radix R (NT threads, layout Lay) holds, in every
+// thread, an element e with lo <= e <= hi: the line kernels use it to drop, at compile time,
+// the loads and stores of slots that fall in the dealiased gap of the packed spectra.
+template <int R, int L, int NT, class Lay>
+constexpr bool ct_slot_within(int i, int lo, int hi) {
+  const int b = i / R, m = i - b * R;
+  constexpr int lr = L / R;
+  for (int tid = 0; tid < NT; ++tid) {
+    int j = 0, t = 0;
+    Lay::template decomp<lr>(tid + b * NT, j, t);
+    const int e = j + m * lr;
+    if (e < lo || e > hi) return false;
+  }
+  return true;
 }
 
 // Register slot i of a pass of radix R holds element e of sequence t.

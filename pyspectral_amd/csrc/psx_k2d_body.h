@@ -61,15 +61,22 @@ __device__ __forceinline__ void body_line2d(const Flow<float>& f, int tile, int 
                   : (e > L - JM ? ((unsigned)FS + base + (L - e)) * 8u : kOob);
   };
 
+  // slots in the dealiased gap JM <= e <= L - JM for every thread hold zeros: no load, and the
+  // first inverse butterflies fold them; the matching outputs of the last forward pass are
+  // never stored, so their arithmetic is dropped
   cx<T> v[E];
   {
     constexpr int R = ct_radix(G.line, true, 0);
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      int e, t;
-      ct_elem<R, L, NT, Lay>(i, tid, e, t);
-      v[i] = zb.load(zoff(e, t));
-    }
+    static_for<0, E>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      if constexpr (ct_slot_within<R, L, NT, Lay>(i, JM, L - JM)) {
+        v[i] = czero<T>();
+      } else {
+        int e, t;
+        ct_elem<R, L, NT, Lay>(i, tid, e, t);
+        v[i] = zb.load(zoff(e, t));
+      }
+    });
   }
   fft_ct<E, true, NT, Lay, G.line>(v, lds, tw_rev, tid);
 
@@ -93,12 +100,14 @@ __device__ __forceinline__ void body_line2d(const Flow<float>& f, int tile, int 
   fft_ct<E, false, NT, Lay, G.line>(v, lds, tw_fwd, tid);
   {
     constexpr int R = ct_radix(G.line, false, G.line.np - 1);
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      int e, t;
-      ct_elem<R, L, NT, Lay>(i, tid, e, t);
-      zb.store(zoff(e, t), v[i]);
-    }
+    static_for<0, E>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      if constexpr (!ct_slot_within<R, L, NT, Lay>(i, JM, L - JM)) {
+        int e, t;
+        ct_elem<R, L, NT, Lay>(i, tid, e, t);
+        zb.store(zoff(e, t), v[i]);
+      }
+    });
   }
 }
 

@@ -50,6 +50,19 @@ __device__ __forceinline__ void z_pack3(int j, cx<float> u, cx<float> v, cx<floa
 }
 
 // ----------------------------------------------------------------------------- z lines
+// Row tiles (G.lineT rows) per work-group of the z-line kernel: a work-group walks TPW
+// consecutive tiles and loads tile k + 1's packed spectra while it transforms tile k, so the
+// loads of one tile overlap the transforms of the previous one (384^3: 729 -> 699 us with four
+// tiles, 716 us with eight).  Grids with fewer tiles keep one per work-group (256^3 gained
+// nothing), and so does the rotational form, whose three input pairs in flight beside its
+// transforms would exceed 256 VGPRs.
+template <int N, bool ROT>
+constexpr int line3d_tpw() {
+  constexpr int tiles = N * N / G3<N>::g.lineT;
+  return ROT || tiles < 16384 ? 1 : 4;
+}
+
+
 template <int N, bool ROT>
 __global__ void __launch_bounds__(G3<N>::g.lineNT, 1) kc_line3d(Flow<float> f, int flags) {
   if (flow_inactive(f.prm, f.nh)) return;
@@ -58,122 +71,188 @@ __global__ void __launch_bounds__(G3<N>::g.lineNT, 1) kc_line3d(Flow<float> f, i
   constexpr int E = G.lineE, NT = G.lineNT, RT = G.lineT, L = N, JM = G.jm, LDW = G.ldw;
   constexpr long long FS = G3<N>::FS;
   constexpr int NF = ROT ? 6 : 5;
+  constexpr int TPW = line3d_tpw<N, ROT>();
   using Lay = RowsCT<L, RT>;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int tid = threadIdx.x;
-  const long long row0 = (long long)blockIdx.x * RT;
   const BufC32T<kc3_pol(N)> wb(f.W, (unsigned long long)NF * FS * 8);
-  const cx<T>* tw_fwd = f.twp + f.prow.twoff[0][0];
-  const cx<T>* tw_rev = f.twp + f.prow.twoff[1][0];
   constexpr int RI = ct_radix(G.line, true, 0);
   constexpr int RO = ct_radix(G.line, false, G.line.np - 1);
 
-  auto pair_off = [&](int fa, int fb, int e, int t) -> unsigned {
-    const unsigned base = (unsigned)((row0 + t) * LDW);
+  // LDS: NREG transform regions of ST elements (the transforms of one direction run together,
+  // fft_ct_multi), then the plan's forward and reversed pass-twiddle tables.  The twiddles are
+  // read from LDS because vector-memory counters retire in order: a twiddle load from global
+  // memory issued after the next tile's loads would wait for them.
+  constexpr int ST = RT * Lay::LP;
+  constexpr int NREG = ROT ? 3 : 2;
+  constexpr int TWF = ct_twsize(G.line, false), TWR = ct_twsize(G.line, true);
+  cx<T>* twl = lds + NREG * ST;
+  {
+    const cx<T>* tw_fwd = f.twp + f.prow.twoff[0][0];
+    const cx<T>* tw_rev = f.twp + f.prow.twoff[1][0];
+    for (int i = tid; i < TWF + TWR; i += NT) twl[i] = i < TWF ? tw_fwd[i] : tw_rev[i - TWF];
+  }
+  __syncthreads();
+
+  // packed spectra of field pair (fa, fb) (or single field fa, raw: the Hermitian extension is
+  // applied when the tile is taken) for rows r0 .. r0 + RT - 1
+  auto pair_off = [&](long long r0, int fa, int fb, int e, int t) -> unsigned {
+    const unsigned base = (unsigned)((r0 + t) * LDW);
     return e < JM ? (unsigned)(fa * FS) + base + e
-                  : (e > L - JM ? (unsigned)(fb * FS) + base + (L - e) : 0x10000000u);
+                  : (e > L - JM ? (unsigned)(fb * FS) + base + (L - e) : kOob / 8u);
   };
-  auto load_pair = [&](cx<T>(&v)[E], int fa, int fb) {
+  auto load_pair = [&](cx<T>(&v)[E], long long r0, int fa, int fb, int tx) {
+    static_for<0, E>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      if constexpr (ct_slot_within<RI, L, NT, Lay>(i, JM, L - JM)) {
+        v[i] = czero<T>();
+      } else {
+        int e, t;
+        ct_elem<RI, L, NT, Lay>(i, tx, e, t);
+        const unsigned o = pair_off(r0, fa, fb, e, t);
+        v[i] = wb.load(o == kOob / 8u ? kOob : o * 8u);
+      }
+    });
+  };
+  auto load_single_raw = [&](cx<T>(&v)[E], long long r0, int fa, int tx) {
+    static_for<0, E>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      if constexpr (ct_slot_within<RI, L, NT, Lay>(i, JM, L - JM)) {
+        v[i] = czero<T>();
+      } else {
+        int e, t;
+        ct_elem<RI, L, NT, Lay>(i, tx, e, t);
+        const unsigned base = (unsigned)((r0 + t) * LDW) + (unsigned)(fa * FS);
+        const int src = e < JM ? e : (e > L - JM ? L - e : -1);
+        v[i] = wb.load(src >= 0 ? (base + src) * 8u : kOob);
+      }
+    });
+  };
+  // Hermitian extension of a real line's spectrum; c2r ignores the imaginary part of the mean
+  auto take_single = [&](cx<T>(&v)[E], int tx) {
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       int e, t;
-      ct_elem<RI, L, NT, Lay>(i, tid, e, t);
-      const unsigned o = pair_off(fa, fb, e, t);
-      v[i] = wb.load(o == 0x10000000u ? kOob : o * 8u);
+      ct_elem<RI, L, NT, Lay>(i, tx, e, t);
+      if (e > L - JM) v[i] = conjg(v[i]);
+      if (e == 0) v[i].y = (T)0;
     }
   };
-  auto load_single = [&](cx<T>(&v)[E], int fa) {
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      int e, t;
-      ct_elem<RI, L, NT, Lay>(i, tid, e, t);
-      const unsigned base = (unsigned)((row0 + t) * LDW) + (unsigned)(fa * FS);
-      const int src = e < JM ? e : (e > L - JM ? L - e : -1);
-      cx<T> z = wb.load(src >= 0 ? (base + src) * 8u : kOob);
-      if (e > L - JM) z = conjg(z);  // Hermitian extension of a real line's spectrum
-      if (e == 0) z.y = (T)0;       // c2r ignores the imaginary part of the mean
-      v[i] = z;
-    }
+  auto store_pair = [&](const cx<T>(&v)[E], long long r0, int fa, int fb, int tx) {
+    static_for<0, E>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      if constexpr (!ct_slot_within<RO, L, NT, Lay>(i, JM, L - JM)) {
+        int e, t;
+        ct_elem<RO, L, NT, Lay>(i, tx, e, t);
+        const unsigned o = pair_off(r0, fa, fb, e, t);
+        wb.store(o == kOob / 8u ? kOob : o * 8u, v[i]);
+      }
+    });
   };
-  auto store_pair = [&](const cx<T>(&v)[E], int fa, int fb) {
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      int e, t;
-      ct_elem<RO, L, NT, Lay>(i, tid, e, t);
-      const unsigned o = pair_off(fa, fb, e, t);
-      wb.store(o == 0x10000000u ? kOob : o * 8u, v[i]);
-    }
+  // (a real field's spectrum is stored for e < JM only)
+  auto store_single = [&](const cx<T>(&v)[E], long long r0, int fa, int tx) {
+    static_for<0, E>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      if constexpr (!ct_slot_within<RO, L, NT, Lay>(i, JM, L)) {
+        int e, t;
+        ct_elem<RO, L, NT, Lay>(i, tx, e, t);
+        const unsigned base = (unsigned)((r0 + t) * LDW) + (unsigned)(fa * FS);
+        wb.store(e < JM ? (base + e) * 8u : kOob, v[i]);
+      }
+    });
   };
-  auto store_single = [&](const cx<T>(&v)[E], int fa) {
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      int e, t;
-      ct_elem<RO, L, NT, Lay>(i, tid, e, t);
-      const unsigned base = (unsigned)((row0 + t) * LDW) + (unsigned)(fa * FS);
-      wb.store(e < JM ? (base + e) * 8u : kOob, v[i]);
+  constexpr int NPF = ROT ? 3 : 2;  // raw input sequences per tile
+  auto fetch = [&](cx<T>(&pf)[NPF][E], long long r0, int tx) {
+    load_pair(pf[0], r0, 0, 1, tx);
+    if constexpr (!ROT) {
+      load_single_raw(pf[1], r0, 2, tx);
+    } else {
+      load_pair(pf[1], r0, 2, 3, tx);
+      load_pair(pf[2], r0, 4, 5, tx);
     }
   };
 
-  // The transforms of one direction run together (fft_ct_multi): their twiddle loads, LDS
-  // exchanges and barriers are shared pass by pass, one LDS region of ST elements each.
-  constexpr int ST = RT * Lay::LP;
-  cx<T> z[3][E];
-  load_pair(z[0], 0, 1);
-  if constexpr (!ROT) {
-    load_single(z[1], 2);
-    fft_ct_multi<2, E, true, NT, Lay, G.line, ST>(z, lds, tw_rev, tid);
-    if (flags & kLineWantMax) {
-      float m[3] = {0.f, 0.f, 0.f};  // |u_i| maxima (exact in fp32)
+  const long long rw0 = (long long)blockIdx.x * TPW * RT;
+  cx<T> pf[NPF][E];
+  fetch(pf, rw0, tid);
+#pragma nounroll
+  for (int k = 0; k < TPW; ++k) {
+    const long long r0 = rw0 + (long long)k * RT;
+    // the thread index and the twiddle base are made opaque per tile: the index arithmetic and
+    // twiddle reads are the same in every tile, and the compiler would otherwise hoist them out
+    // of the loop and keep them in registers across it
+    int tk = tid, twz = 0;
+    asm volatile("" : "+v"(tk), "+v"(twz));
+    const cx<T>* tw_fwd_l = twl + twz;
+    const cx<T>* tw_rev_l = twl + TWF + twz;
+    cx<T> z[3][E];
+#pragma unroll
+    for (int s = 0; s < NPF; ++s) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) z[s][i] = pf[s][i];
+    }
+    if constexpr (!ROT) {
+      if (k + 1 < TPW) fetch(pf, r0 + RT, tk);
+      take_single(z[1], tk);
+      fft_ct_multi<2, E, true, NT, Lay, G.line, ST>(z, lds, tw_rev_l, tk);
+      if (flags & kLineWantMax) {
+        float m[3] = {0.f, 0.f, 0.f};  // |u_i| maxima (exact in fp32)
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+          m[0] = fmaxf(m[0], abs_or_inf(z[0][i].x));
+          m[1] = fmaxf(m[1], abs_or_inf(z[0][i].y));
+          m[2] = fmaxf(m[2], abs_or_inf(z[1][i].x));
+        }
+        double md[3] = {(double)m[0], (double)m[1], (double)m[2]};
+        block_max_to<3>(md, f.stats, reinterpret_cast<double*>(smem_raw));
+      }
+      if (flags & kLineMaxOnly) continue;
+      // the fifth (real) product v w first, alone in region 0, while only u, v, w are live;
+      // then the two packed pairs together
+      cx<T> z2[1][E];
+#pragma unroll
+      for (int i = 0; i < E; ++i) z2[0][i] = mkc(z[0][i].y * z[1][i].x, (T)0);
+      fft_ct_multi<1, E, false, NT, Lay, G.line, ST>(z2, lds, tw_fwd_l, tk);
+      store_single(z2[0], r0, 4, tk);
+      const T third = (T)1 / (T)3;
 #pragma unroll
       for (int i = 0; i < E; ++i) {
-        m[0] = fmaxf(m[0], abs_or_inf(z[0][i].x));
-        m[1] = fmaxf(m[1], abs_or_inf(z[0][i].y));
-        m[2] = fmaxf(m[2], abs_or_inf(z[1][i].x));
+        const T u = z[0][i].x, v = z[0][i].y, w = z[1][i].x;
+        const T q = (u * u + v * v + w * w) * third;
+        z[0][i] = mkc(u * u - q, v * v - q);
+        z[1][i] = mkc(u * v, u * w);
       }
-      double md[3] = {(double)m[0], (double)m[1], (double)m[2]};
-      block_max_to<3>(md, f.stats, reinterpret_cast<double*>(smem_raw));
-    }
-    if (flags & kLineMaxOnly) return;
-    const T third = (T)1 / (T)3;
+      fft_ct_multi<2, E, false, NT, Lay, G.line, ST>(z, lds, tw_fwd_l, tk);
+      store_pair(z[0], r0, 0, 1, tk);
+      store_pair(z[1], r0, 2, 3, tk);
+    } else {
+      fft_ct_multi<3, E, true, NT, Lay, G.line, ST>(z, lds, tw_rev_l, tk);
+      if (flags & kLineWantMax) {
+        float m[3] = {0.f, 0.f, 0.f};  // |u_i| maxima (exact in fp32)
 #pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const T u = z[0][i].x, v = z[0][i].y, w = z[1][i].x;
-      const T q = (u * u + v * v + w * w) * third;
-      z[0][i] = mkc(u * u - q, v * v - q);
-      z[1][i] = mkc(u * v, u * w);
-      z[2][i] = mkc(v * w, (T)0);
-    }
-    fft_ct_multi<3, E, false, NT, Lay, G.line, ST>(z, lds, tw_fwd, tid);
-    store_pair(z[0], 0, 1);
-    store_pair(z[1], 2, 3);
-    store_single(z[2], 4);
-  } else {
-    load_pair(z[1], 2, 3);
-    load_pair(z[2], 4, 5);
-    fft_ct_multi<3, E, true, NT, Lay, G.line, ST>(z, lds, tw_rev, tid);
-    if (flags & kLineWantMax) {
-      float m[3] = {0.f, 0.f, 0.f};  // |u_i| maxima (exact in fp32)
+        for (int i = 0; i < E; ++i) {
+          m[0] = fmaxf(m[0], abs_or_inf(z[0][i].x));
+          m[1] = fmaxf(m[1], abs_or_inf(z[0][i].y));
+          m[2] = fmaxf(m[2], abs_or_inf(z[1][i].x));
+        }
+        double md[3] = {(double)m[0], (double)m[1], (double)m[2]};
+        block_max_to<3>(md, f.stats, reinterpret_cast<double*>(smem_raw));
+      }
+      if (flags & kLineMaxOnly) continue;
 #pragma unroll
       for (int i = 0; i < E; ++i) {
-        m[0] = fmaxf(m[0], abs_or_inf(z[0][i].x));
-        m[1] = fmaxf(m[1], abs_or_inf(z[0][i].y));
-        m[2] = fmaxf(m[2], abs_or_inf(z[1][i].x));
+        const T u = z[0][i].x, v = z[0][i].y, w = z[1][i].x;
+        const T ox = z[1][i].y, oy = z[2][i].x, oz = z[2][i].y;
+        z[0][i] = mkc(v * oz - w * oy, w * ox - u * oz);
+        z[1][i] = mkc(u * oy - v * ox, (T)0);
       }
-      double md[3] = {(double)m[0], (double)m[1], (double)m[2]};
-      block_max_to<3>(md, f.stats, reinterpret_cast<double*>(smem_raw));
+      // the next tile's three pairs are fetched once the inputs are consumed
+      if (k + 1 < TPW) fetch(pf, r0 + RT, tk);
+      fft_ct_multi<2, E, false, NT, Lay, G.line, ST>(z, lds, tw_fwd_l, tk);
+      store_pair(z[0], r0, 0, 1, tk);
+      store_single(z[1], r0, 2, tk);
     }
-    if (flags & kLineMaxOnly) return;
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      const T u = z[0][i].x, v = z[0][i].y, w = z[1][i].x;
-      const T ox = z[1][i].y, oy = z[2][i].x, oz = z[2][i].y;
-      z[0][i] = mkc(v * oz - w * oy, w * ox - u * oz);
-      z[1][i] = mkc(u * oy - v * ox, (T)0);
-    }
-    fft_ct_multi<2, E, false, NT, Lay, G.line, ST>(z, lds, tw_fwd, tid);
-    store_pair(z[0], 0, 1);
-    store_single(z[1], 2);
   }
 }
 
@@ -525,11 +604,14 @@ static hipError_t ct3_launch(const Flow<float>& f, int which, int arg, hipStream
       auto fn = kc_line3d<N, ROT>;
       static const bool attr_ = set_lds_max((const void*)fn);
       (void)attr_;
-      // three sequence regions (the transforms of one direction run together)
-      size_t lds = (size_t)3 * G.lineT * RowsCT<N, G.lineT>::LP * 8;
+      // transform regions (three rotational, two Basdevant) and the two twiddle tables
+      size_t lds = ((size_t)(ROT ? 3 : 2) * G.lineT * RowsCT<N, G.lineT>::LP +
+                    ct_twsize(G.line, false) + ct_twsize(G.line, true)) * 8;
       const size_t red = (size_t)3 * G.lineNT * sizeof(double);
       if (lds < red) lds = red;
-      hipLaunchKernelGGL(fn, dim3(N * N / G.lineT), dim3(G.lineNT), lds, st, f, arg);
+      static_assert((N * N / G.lineT) % line3d_tpw<N, ROT>() == 0, "tiles per work-group");
+      hipLaunchKernelGGL(fn, dim3(N * N / G.lineT / line3d_tpw<N, ROT>()), dim3(G.lineNT), lds, st, f,
+                         arg);
       break;
     }
     case 1:
