@@ -481,9 +481,8 @@ def single_bench(a: argparse.Namespace) -> dict[str, Any]:
     torch.cuda.empty_cache()
 
     kb = kernel_bytes(n, lay.jm, 8)
-    traffic, tnote = measured_traffic("line", f"2d_{n}", "2d")
-    roof = roofline_entry(per_kernel, kb, traffic if roof_kernel(per_kernel, kb) == "line"
-                          else None, tnote)
+    traffic, tnote = measured_traffic(roof_kernel(per_kernel, kb), f"2d_{n}", "2d")
+    roof = roofline_entry(per_kernel, kb, traffic, tnote)
     step_bytes = sum(kb[k] * per_kernel[k]["launches"] for k in per_kernel if k in kb) / a.steps
     tm = traffic_model(n, 2, "basdevant", "float32")
     value = n * n / t_step / 1e6
@@ -558,9 +557,8 @@ def tgv3d_bench(a: argparse.Namespace, n: int, cpu: bool) -> dict[str, Any]:
     del s
     torch.cuda.empty_cache()
     kb = kernel_bytes_3d(n, lay.jm, lay.ldw, False, 8)
-    traffic, tnote = measured_traffic("line", f"3d_{n}", "3d")
-    roof = roofline_entry(per_kernel, kb, traffic if roof_kernel(per_kernel, kb) == "line"
-                          else None, tnote)
+    traffic, tnote = measured_traffic(roof_kernel(per_kernel, kb), f"3d_{n}", "3d")
+    roof = roofline_entry(per_kernel, kb, traffic, tnote)
     step_bytes = sum(kb[k] * per_kernel[k]["launches"] for k in per_kernel if k in kb) / a.steps
     tm = traffic_model(n, 3, "basdevant", "float32")
     value = n ** 3 / t_step / 1e6
