@@ -42,7 +42,21 @@ struct ColsCT {
   }
 };
 
-template <int E, int R, bool INV, int L, int Ns, int NT, class Lay, typename T>
+// Barrier of an LDS exchange.  WS: the exchange stays within one wave (a tile of at most 64
+// threads inside a 64-thread group): the workgroup-scope fences of a block barrier without the
+// s_barrier, so the other waves of the work-group (other tiles) are not waited for.
+template <bool WS>
+__device__ __forceinline__ void ct_sync() {
+  if constexpr (WS) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  } else {
+    __syncthreads();
+  }
+}
+
+template <int E, int R, bool INV, int L, int Ns, int NT, class Lay, bool WS = false, typename T>
 __device__ __forceinline__ void ct_pass(cx<T> (&v)[E], cx<T>* lds, bool from_lds, bool to_lds,
                                         const cx<T>* __restrict__ tw, int tid) {
   static_assert(E % R == 0, "radix must divide the per-thread element count");
@@ -72,7 +86,7 @@ __device__ __forceinline__ void ct_pass(cx<T> (&v)[E], cx<T>* lds, bool from_lds
     dft<R, INV>(&v[b * R]);
   }
   if (to_lds) {
-    __syncthreads();
+    ct_sync<WS>();
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       int j, t;
@@ -82,13 +96,13 @@ __device__ __forceinline__ void ct_pass(cx<T> (&v)[E], cx<T>* lds, bool from_lds
 #pragma unroll
       for (int q = 0; q < R; ++q) lds[Lay::addr(base + q * Ns, t)] = v[b * R + q];
     }
-    __syncthreads();
+    ct_sync<WS>();
   }
 }
 
 // Whole transform; tw_order points at the start of this plan's forward (INV = false) or
 // reversed-order (INV = true) twiddle table.
-template <int E, bool INV, int NT, class Lay, CtPlan P, typename T>
+template <int E, bool INV, int NT, class Lay, CtPlan P, bool WS = false, typename T>
 __device__ __forceinline__ void fft_ct(cx<T> (&v)[E], cx<T>* lds,
                                        const cx<T>* __restrict__ tw_order, int tid) {
   static_for<0, P.np>([&](auto qc) {
@@ -96,7 +110,7 @@ __device__ __forceinline__ void fft_ct(cx<T> (&v)[E], cx<T>* lds,
     constexpr int R = ct_radix(P, INV, q);
     constexpr int Ns = ct_ns(P, INV, q);
     constexpr int off = ct_twoff(P, INV, q);
-    ct_pass<E, R, INV, P.L, Ns, NT, Lay>(v, lds, q > 0, q < P.np - 1, tw_order + off, tid);
+    ct_pass<E, R, INV, P.L, Ns, NT, Lay, WS>(v, lds, q > 0, q < P.np - 1, tw_order + off, tid);
   });
 }
 
@@ -107,7 +121,7 @@ __device__ __forceinline__ void fft_ct(cx<T> (&v)[E], cx<T>* lds,
 // latency; together, the loads and LDS accesses of the NS sequences are in flight at once.
 // v[0 .. NS-1] are transformed (NS <= NA).
 template <int NS, int NA, int E, int R, bool INV, int L, int Ns, int NT, class Lay, int STRIDE,
-          bool FROM, bool TO, typename T>
+          bool FROM, bool TO, bool WS = false, typename T>
 __device__ __forceinline__ void ct_pass_multi(cx<T> (&v)[NA][E], cx<T>* lds,
                                               const cx<T>* __restrict__ tw, int tid) {
   static_assert(E % R == 0, "radix must divide the per-thread element count");
@@ -144,7 +158,7 @@ __device__ __forceinline__ void ct_pass_multi(cx<T> (&v)[NA][E], cx<T>* lds,
     for (int s = 0; s < NS; ++s) dft<R, INV>(&v[s][b * R]);
   }
   if constexpr (TO) {
-    __syncthreads();
+    ct_sync<WS>();
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       int j, t;
@@ -157,11 +171,12 @@ __device__ __forceinline__ void ct_pass_multi(cx<T> (&v)[NA][E], cx<T>* lds,
         for (int q = 0; q < R; ++q) lds[s * STRIDE + Lay::addr(base + q * Ns, t)] = v[s][b * R + q];
       }
     }
-    __syncthreads();
+    ct_sync<WS>();
   }
 }
 
-template <int NS, int E, bool INV, int NT, class Lay, CtPlan P, int STRIDE, int NA, typename T>
+template <int NS, int E, bool INV, int NT, class Lay, CtPlan P, int STRIDE, bool WS = false,
+          int NA, typename T>
 __device__ __forceinline__ void fft_ct_multi(cx<T> (&v)[NA][E], cx<T>* lds,
                                              const cx<T>* __restrict__ tw_order, int tid) {
   static_for<0, P.np>([&](auto qc) {
@@ -169,7 +184,7 @@ __device__ __forceinline__ void fft_ct_multi(cx<T> (&v)[NA][E], cx<T>* lds,
     constexpr int R = ct_radix(P, INV, q);
     constexpr int Ns = ct_ns(P, INV, q);
     constexpr int off = ct_twoff(P, INV, q);
-    ct_pass_multi<NS, NA, E, R, INV, P.L, Ns, NT, Lay, STRIDE, (q > 0), (q < P.np - 1)>(
+    ct_pass_multi<NS, NA, E, R, INV, P.L, Ns, NT, Lay, STRIDE, (q > 0), (q < P.np - 1), WS>(
         v, lds, tw_order + off, tid);
   });
 }

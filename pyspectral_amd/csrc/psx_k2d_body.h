@@ -259,7 +259,9 @@ __device__ __forceinline__ void body_spec2d(const Flow<float>& f, int tile, int 
 // Outer-axis transforms of column tile `tile` (inner index tile / NJT, columns
 // (tile % NJT) * colT ..) of flow f; outNT threads, tid in [0, outNT).  MS as in
 // body_spec2d (two LDS regions of n1 * colT elements).
-template <Geo2D G, bool INV, bool MS = false>
+// WS: the tile's threads lie within one wave, so its exchanges use wave-level syncs (several
+// tiles share a work-group without waiting on each other).
+template <Geo2D G, bool INV, bool MS = false, bool WS = false>
 __device__ __forceinline__ void body_outer2d(const Flow<float>& f, int tile, int tid,
                                              unsigned char* smem_raw) {
   using T = float;
@@ -293,10 +295,10 @@ __device__ __forceinline__ void body_outer2d(const Flow<float>& f, int tile, int
       }
     }
     if constexpr (MS) {
-      fft_ct_multi<2, E, INV, NT, Lay, G.outer, L * CT>(vv, lds, tw_o, tid);
+      fft_ct_multi<2, E, INV, NT, Lay, G.outer, L * CT, WS>(vv, lds, tw_o, tid);
     } else {
-      fft_ct<E, INV, NT, Lay, G.outer>(v0, lds, tw_o, tid);
-      fft_ct<E, INV, NT, Lay, G.outer>(v1, lds, tw_o, tid);
+      fft_ct<E, INV, NT, Lay, G.outer, WS>(v0, lds, tw_o, tid);
+      fft_ct<E, INV, NT, Lay, G.outer, WS>(v1, lds, tw_o, tid);
     }
 #pragma unroll
     for (int i = 0; i < E; ++i) {

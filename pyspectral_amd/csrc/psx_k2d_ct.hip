@@ -4,6 +4,7 @@
 // line-spectrum convention are in psx_k2d_body.h.
 #include "psx_k2d_body.h"
 
+
 namespace psx {
 
 template <int N>
@@ -35,12 +36,31 @@ __global__ void __launch_bounds__(G2<N>::g.specNT, 2) kc_spec2d(Flow<float> f) {
   body_spec2d<G2<N>::g, stage, kMs2<N>>(f, blockIdx.x, threadIdx.x, smem_raw);
 }
 
+// Outer-pass tiles per work-group: tiles of at most one wave (n1 * colT / outE <= 64 threads,
+// dividing 64) are packed kOuterS to a work-group, each exchanging through its own LDS with
+// wave-level syncs; one-wave work-groups had averaged 1.3 resident waves per SIMD.  8192^2:
+// forward 152.7 -> 147.2 us with 4 tiles (158 with 8), inverse 147.5 -> 128.0 us with 8
+// (131.9 with 4).
 template <int N, bool INV>
-__global__ void __launch_bounds__(G2<N>::g.outNT > 0 ? G2<N>::g.outNT : 64, 1)
+constexpr int kOuterS = (G2<N>::g.n1 > 1 && G2<N>::g.outNT <= 64 && 64 % G2<N>::g.outNT == 0 &&
+                         (G2<N>::g.n2 * (G2<N>::g.ldw / G2<N>::g.colT)) % (INV ? 8 : 4) == 0)
+                            ? (INV ? 8 : 4)
+                            : 1;
+
+template <int N, bool INV>
+__global__ void __launch_bounds__((G2<N>::g.outNT > 0 ? G2<N>::g.outNT : 64) * (kOuterS<N, INV>), 1)
     kc_outer2d(Flow<float> f) {
   if (flow_inactive(f.prm, f.nh)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  body_outer2d<G2<N>::g, INV, INV>(f, blockIdx.x, threadIdx.x, smem_raw);
+  constexpr Geo2D G = G2<N>::g;
+  constexpr int S = kOuterS<N, INV>;
+  if constexpr (S > 1) {
+    const int sub = threadIdx.x / G.outNT;
+    body_outer2d<G, INV, INV, true>(f, blockIdx.x * S + sub, threadIdx.x - sub * G.outNT,
+                                    smem_raw + (size_t)sub * (INV ? 2 : 1) * G.n1 * G.colT * 8);
+  } else {
+    body_outer2d<G, INV, INV>(f, blockIdx.x, threadIdx.x, smem_raw);
+  }
 }
 
 // ------------------------------------------------------------------------------ launchers
@@ -86,13 +106,18 @@ static hipError_t ct_launch(const Flow<float>& f, int which, int arg, hipStream_
     case 2:
     case 3: {
       if constexpr (G.n1 > 1) {
-        const size_t lds = (size_t)(which == 3 ? 2 : 1) * G.n1 * G.colT * 8;
         if (which == 2) {
+          constexpr int S = kOuterS<N, false>;
+          const size_t lds = (size_t)S * G.n1 * G.colT * 8;
           auto fn = kc_outer2d<N, false>;
-          hipLaunchKernelGGL(fn, dim3(G.n2 * NJT), dim3(G.outNT), lds, st, f);
+          hipLaunchKernelGGL(fn, dim3(G.n2 * NJT / S), dim3(G.outNT * S), lds, st, f);
         } else {
+          constexpr int S = kOuterS<N, true>;
+          const size_t lds = (size_t)S * 2 * G.n1 * G.colT * 8;  // two sequence regions
           auto fn = kc_outer2d<N, true>;
-          hipLaunchKernelGGL(fn, dim3(G.n2 * NJT), dim3(G.outNT), lds, st, f);
+          static const bool attr_ = set_lds_max((const void*)fn);
+          (void)attr_;
+          hipLaunchKernelGGL(fn, dim3(G.n2 * NJT / S), dim3(G.outNT * S), lds, st, f);
         }
       } else {
         return hipErrorInvalidValue;
