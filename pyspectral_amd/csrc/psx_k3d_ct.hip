@@ -287,66 +287,94 @@ __global__ void __launch_bounds__(G3<N>::g.yNT, 1) kc_ypass3d(Flow<float> f) {
   const int x = blockIdx.x / NJT;
   const int j0 = (blockIdx.x - x * NJT) * CT;
   const long long rb = (long long)x * N;
-  const cx<T>* tw_o = f.twp + f.py.twoff[INV ? 1 : 0][0];
   constexpr int R0 = ct_radix(G.yp, INV, 0);
   constexpr int RL = ct_radix(G.yp, INV, G.yp.np - 1);
-  // packed pairs and single fields of this direction
+  // packed pairs and single fields of this direction, transformed group by group (a pair's two
+  // transforms together in two LDS regions); the next group's loads are issued before the
+  // current group's transforms, so they overlap.  The pass twiddles are read from an LDS copy:
+  // vector-memory counters retire in order, and a twiddle load from global memory issued after
+  // the next group's loads would wait for them.
   constexpr int NPAIR = INV ? (ROT ? 3 : 1) : (ROT ? 1 : 2);
   constexpr int SINGLE = INV ? (ROT ? -1 : 2) : (ROT ? 2 : 4);
+  constexpr int NG = NPAIR + (SINGLE >= 0 ? 1 : 0);
+  constexpr int TW = ct_twsize(G.yp, INV);
+  cx<T>* twl = lds + 2 * L * CT;
+  {
+    const cx<T>* tw_o = f.twp + f.py.twoff[INV ? 1 : 0][0];
+    for (int i = tid; i < TW; i += NT) twl[i] = tw_o[i];
+  }
+  __syncthreads();
 
-  static_for<0, NPAIR>([&](auto pc) {
-    constexpr int fa = 2 * decltype(pc)::value, fb = fa + 1;
-    cx<T> vv[2][E];
-    cx<T>(&va)[E] = vv[0];
-    cx<T>(&vb)[E] = vv[1];
+  // raw loads of group g (pair fields 2g, 2g + 1, or the single field)
+  auto load_group = [&](auto gc, cx<T>(&g)[2][E]) {
+    constexpr int gi = decltype(gc)::value;
 #pragma unroll
     for (int i = 0; i < E; ++i) {
       int e, t;
       ct_elem<R0, L, NT, Lay>(i, tid, e, t);
       const long long a = (rb + e) * LDW + j0 + t;
-      const cx<T> z = load_w<N>(f.W + fa * FS + a);
-      const cx<T> zm = load_w<N>(f.W + fb * FS + a);
-      if constexpr (INV) {
-        va[i] = z;
-        vb[i] = zm;
+      if constexpr (gi < NPAIR) {
+        g[0][i] = load_w<N>(f.W + 2 * gi * FS + a);
+        g[1][i] = load_w<N>(f.W + (2 * gi + 1) * FS + a);
       } else {
-        z_split3(z, j0 + t == 0 ? z : zm, va[i], vb[i]);
+        g[0][i] = load_w<N>(f.W + SINGLE * FS + a);
       }
     }
-    // the pair's two transforms together (two LDS regions)
-    fft_ct_multi<2, E, INV, NT, Lay, G.yp, L * CT>(vv, lds, tw_o, tid);
+  };
+  auto run_group = [&](auto gc, cx<T>(&g)[2][E]) {
+    constexpr int gi = decltype(gc)::value;
+    if constexpr (gi < NPAIR) {
+      constexpr int fa = 2 * gi, fb = fa + 1;
+      if constexpr (!INV) {
 #pragma unroll
-    for (int i = 0; i < E; ++i) {
-      int e, t;
-      ct_elem<RL, L, NT, Lay>(i, tid, e, t);
-      const long long a = (rb + e) * LDW + j0 + t;
-      if constexpr (INV) {
-        cx<T> z, zm;
-        z_pack3(j0 + t, va[i], vb[i], z, zm);
-        f.W[fa * FS + a] = z;
-        f.W[fb * FS + a] = zm;
-      } else {
-        f.W[fa * FS + a] = va[i];
-        f.W[fb * FS + a] = vb[i];
+        for (int i = 0; i < E; ++i) {
+          int e, t;
+          ct_elem<R0, L, NT, Lay>(i, tid, e, t);
+          const cx<T> z = g[0][i], zm = g[1][i];
+          z_split3(z, j0 + t == 0 ? z : zm, g[0][i], g[1][i]);
+        }
+      }
+      fft_ct_multi<2, E, INV, NT, Lay, G.yp, L * CT>(g, lds, twl, tid);
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        int e, t;
+        ct_elem<RL, L, NT, Lay>(i, tid, e, t);
+        const long long a = (rb + e) * LDW + j0 + t;
+        if constexpr (INV) {
+          cx<T> z, zm;
+          z_pack3(j0 + t, g[0][i], g[1][i], z, zm);
+          f.W[fa * FS + a] = z;
+          f.W[fb * FS + a] = zm;
+        } else {
+          f.W[fa * FS + a] = g[0][i];
+          f.W[fb * FS + a] = g[1][i];
+        }
+      }
+    } else {
+      fft_ct_multi<1, E, INV, NT, Lay, G.yp, L * CT>(g, lds, twl, tid);
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        int e, t;
+        ct_elem<RL, L, NT, Lay>(i, tid, e, t);
+        f.W[SINGLE * FS + (rb + e) * LDW + j0 + t] = g[0][i];
+      }
+    }
+  };
+  cx<T> cur[2][E];
+  load_group(std::integral_constant<int, 0>{}, cur);
+  static_for<0, NG>([&](auto gc) {
+    constexpr int gi = decltype(gc)::value;
+    cx<T> nxt[2][E];
+    if constexpr (gi + 1 < NG) load_group(std::integral_constant<int, gi + 1>{}, nxt);
+    run_group(gc, cur);
+    if constexpr (gi + 1 < NG) {
+#pragma unroll
+      for (int i = 0; i < E; ++i) {
+        cur[0][i] = nxt[0][i];
+        cur[1][i] = nxt[1][i];
       }
     }
   });
-  if constexpr (SINGLE >= 0) {
-    cx<T> v[E];
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      int e, t;
-      ct_elem<R0, L, NT, Lay>(i, tid, e, t);
-      v[i] = load_w<N>(f.W + SINGLE * FS + (rb + e) * LDW + j0 + t);
-    }
-    fft_ct<E, INV, NT, Lay, G.yp>(v, lds, tw_o, tid);
-#pragma unroll
-    for (int i = 0; i < E; ++i) {
-      int e, t;
-      ct_elem<RL, L, NT, Lay>(i, tid, e, t);
-      f.W[SINGLE * FS + (rb + e) * LDW + j0 + t] = v[i];
-    }
-  }
 }
 
 // ------------------------------------------------------------------ x pass with physics
@@ -616,7 +644,11 @@ static hipError_t ct3_launch(const Flow<float>& f, int which, int arg, hipStream
     }
     case 1:
     case 3: {
-      const size_t lds = (size_t)2 * N * G.T * 8;  // two sequence regions
+      // two sequence regions and the pass twiddles of the direction
+      const size_t lds = ((size_t)2 * N * G.T + ct_twsize(G.yp, which == 3)) * 8;
+      static const bool attr_ = set_lds_max((const void*)kc_ypass3d<N, ROT, false>) &&
+                                set_lds_max((const void*)kc_ypass3d<N, ROT, true>);
+      (void)attr_;
       if (which == 1)
         hipLaunchKernelGGL((kc_ypass3d<N, ROT, false>), dim3(N * NJT), dim3(G.yNT), lds, st, f);
       else
