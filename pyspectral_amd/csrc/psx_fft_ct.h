@@ -114,6 +114,24 @@ __device__ __forceinline__ void fft_ct(cx<T> (&v)[E], cx<T>* lds,
   });
 }
 
+// As fft_ct, with the twiddles of passes q < QG read from tw_near (a copy of the start of the
+// order's table, e.g. in LDS) and those of the later passes from tw_far.
+template <int E, bool INV, int NT, class Lay, CtPlan P, int QG, typename T>
+__device__ __forceinline__ void fft_ct_tw2(cx<T> (&v)[E], cx<T>* lds,
+                                           const cx<T>* __restrict__ tw_near,
+                                           const cx<T>* __restrict__ tw_far, int tid) {
+  static_for<0, P.np>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int R = ct_radix(P, INV, q);
+    constexpr int Ns = ct_ns(P, INV, q);
+    constexpr int off = ct_twoff(P, INV, q);
+    if constexpr (q < QG)
+      ct_pass<E, R, INV, P.L, Ns, NT, Lay>(v, lds, q > 0, q < P.np - 1, tw_near + off, tid);
+    else
+      ct_pass<E, R, INV, P.L, Ns, NT, Lay>(v, lds, q > 0, q < P.np - 1, tw_far + off, tid);
+  });
+}
+
 // NS independent transforms of the same plan advanced pass by pass together: sequence s
 // exchanges through its own LDS region at lds + s * STRIDE (elements), each pass loads a
 // twiddle once for all NS sequences, and the NS exchanges of a pass share one pair of

@@ -40,7 +40,15 @@ __device__ __forceinline__ void z_pack(int j, cx<float> u, cx<float> v, cx<float
 
 // --------------------------------------------------------------------------- line kernel
 // Rows [tile * lineT, (tile + 1) * lineT) of flow f; lineNT threads, tid in [0, lineNT).
+// TWL: the twiddles of every pass but the last of each direction are read from an LDS copy
+// after the row tiles (the caller allocates twl_line2d_elems<G>() more elements); the last
+// passes' tables (60 KB each at 8192) stay in global memory.
 template <Geo2D G>
+constexpr int twl_line2d_elems() {
+  return ct_twoff(G.line, false, G.line.np - 1) + ct_twoff(G.line, true, G.line.np - 1);
+}
+
+template <Geo2D G, bool TWL = false>
 __device__ __forceinline__ void body_line2d(const Flow<float>& f, int tile, int tid, int flags,
                                             unsigned char* smem_raw) {
   using T = float;
@@ -53,6 +61,14 @@ __device__ __forceinline__ void body_line2d(const Flow<float>& f, int tile, int 
   const BufC32 zb(f.W, (unsigned long long)(2 * FS) * 8);
   const cx<T>* tw_fwd = f.twp + f.prow.twoff[0][0];
   const cx<T>* tw_rev = f.twp + f.prow.twoff[1][0];
+  constexpr int QG = G.line.np - 1;
+  constexpr int TWNF = ct_twoff(G.line, false, QG);
+  cx<T>* twl = lds + RT * Lay::LP;
+  if constexpr (TWL) {
+    constexpr int TWN = twl_line2d_elems<G>();
+    for (int i = tid; i < TWN; i += NT) twl[i] = i < TWNF ? tw_fwd[i] : tw_rev[i - TWNF];
+    __syncthreads();
+  }
   // byte offset of element e of row-in-tile t: field 0 for e < JM, field 1 (mirrored) for
   // e > L - JM, out of range (zero / dropped) in between
   auto zoff = [&](int e, int t) -> unsigned {
@@ -78,7 +94,11 @@ __device__ __forceinline__ void body_line2d(const Flow<float>& f, int tile, int 
       }
     });
   }
-  fft_ct<E, true, NT, Lay, G.line>(v, lds, tw_rev, tid);
+  if constexpr (TWL) {
+    fft_ct_tw2<E, true, NT, Lay, G.line, QG>(v, lds, twl + TWNF, tw_rev, tid);
+  } else {
+    fft_ct<E, true, NT, Lay, G.line>(v, lds, tw_rev, tid);
+  }
 
   if (flags & kLineWantMax) {
     float m[2] = {0.f, 0.f};  // |u_i| maxima (exact in fp32)
@@ -97,7 +117,11 @@ __device__ __forceinline__ void body_line2d(const Flow<float>& f, int tile, int 
     const T u = v[i].x, w = v[i].y;
     v[i] = mkc((w - u) * (w + u), u * w);
   }
-  fft_ct<E, false, NT, Lay, G.line>(v, lds, tw_fwd, tid);
+  if constexpr (TWL) {
+    fft_ct_tw2<E, false, NT, Lay, G.line, QG>(v, lds, twl, tw_fwd, tid);
+  } else {
+    fft_ct<E, false, NT, Lay, G.line>(v, lds, tw_fwd, tid);
+  }
   {
     constexpr int R = ct_radix(G.line, false, G.line.np - 1);
     static_for<0, E>([&](auto ic) {
@@ -116,7 +140,9 @@ __device__ __forceinline__ void body_line2d(const Flow<float>& f, int tile, int 
 // of flow f; specNT threads, tid in [0, specNT).  MS: the two fields' transforms of one
 // direction run together (fft_ct_multi, two LDS regions of n2 * colT elements) instead of one
 // after the other in one region.
-template <Geo2D G, int stage, bool MS = false>
+// TWL: the pass twiddles are read from an LDS copy after the transform regions (the caller
+// allocates it; twl_spec2d_elems elements).
+template <Geo2D G, int stage, bool MS = false, bool TWL = false>
 __device__ __forceinline__ void body_spec2d(const Flow<float>& f, int tile, int tid,
                                             unsigned char* smem_raw) {
   using T = float;
@@ -135,6 +161,14 @@ __device__ __forceinline__ void body_spec2d(const Flow<float>& f, int tile, int 
   cx<T>* W1 = f.W + FS;
   const cx<T>* tw_fwd = f.twp + f.pcol.twoff[0][0];
   const cx<T>* tw_rev = f.twp + f.pcol.twoff[1][0];
+  if constexpr (TWL) {
+    constexpr int TWF = ct_twsize(G.spec, false), TWR = ct_twsize(G.spec, true);
+    cx<T>* twl = lds + (MS ? 2 : 1) * L * CT;
+    for (int i = tid; i < TWF + TWR; i += NT) twl[i] = i < TWF ? tw_fwd[i] : tw_rev[i - TWF];
+    __syncthreads();
+    tw_fwd = twl;
+    tw_rev = twl + TWF;
+  }
 
   cx<T> vv[2][E];
   cx<T>(&va)[E] = vv[0];
@@ -261,9 +295,11 @@ __device__ __forceinline__ void body_spec2d(const Flow<float>& f, int tile, int 
 // body_spec2d (two LDS regions of n1 * colT elements).
 // WS: the tile's threads lie within one wave, so its exchanges use wave-level syncs (several
 // tiles share a work-group without waiting on each other).
-template <Geo2D G, bool INV, bool MS = false, bool WS = false>
+// TWL: tw_lds holds this direction's pass-twiddle table in LDS (copied by the caller).
+template <Geo2D G, bool INV, bool MS = false, bool WS = false, bool TWL = false>
 __device__ __forceinline__ void body_outer2d(const Flow<float>& f, int tile, int tid,
-                                             unsigned char* smem_raw) {
+                                             unsigned char* smem_raw,
+                                             const cx<float>* tw_lds = nullptr) {
   using T = float;
   constexpr int N = G.n;
   if constexpr (G.n1 > 1) {
@@ -274,7 +310,7 @@ __device__ __forceinline__ void body_outer2d(const Flow<float>& f, int tile, int
     cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
     const int x2 = tile / NJT;
     const int j0 = (tile - x2 * NJT) * CT;
-    const cx<T>* tw_o = f.twp + f.pa.twoff[INV ? 1 : 0][0];
+    const cx<T>* tw_o = TWL ? tw_lds : f.twp + f.pa.twoff[INV ? 1 : 0][0];
     constexpr int R0 = ct_radix(G.outer, INV, 0);
     constexpr int RL = ct_radix(G.outer, INV, G.outer.np - 1);
     cx<T> vv[2][E];

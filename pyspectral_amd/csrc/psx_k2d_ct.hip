@@ -26,14 +26,14 @@ template <int N>
 __global__ void __launch_bounds__(G2<N>::g.lineNT, 1) kc_line2d(Flow<float> f, int flags) {
   if (flow_inactive(f.prm, f.nh)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  body_line2d<G2<N>::g>(f, blockIdx.x, threadIdx.x, flags, smem_raw);
+  body_line2d<G2<N>::g, true>(f, blockIdx.x, threadIdx.x, flags, smem_raw);
 }
 
 template <int N, int stage>
 __global__ void __launch_bounds__(G2<N>::g.specNT, 2) kc_spec2d(Flow<float> f) {
   if (flow_inactive(f.prm, f.nh)) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  body_spec2d<G2<N>::g, stage, kMs2<N>>(f, blockIdx.x, threadIdx.x, smem_raw);
+  body_spec2d<G2<N>::g, stage, kMs2<N>, true>(f, blockIdx.x, threadIdx.x, smem_raw);
 }
 
 // Outer-pass tiles per work-group: tiles of at most one wave (n1 * colT / outE <= 64 threads,
@@ -54,12 +54,23 @@ __global__ void __launch_bounds__((G2<N>::g.outNT > 0 ? G2<N>::g.outNT : 64) * (
   extern __shared__ __align__(16) unsigned char smem_raw[];
   constexpr Geo2D G = G2<N>::g;
   constexpr int S = kOuterS<N, INV>;
+  // the inverse pass reads its pass twiddles from one LDS copy for the work-group's tiles
+  // (8192^2: 128 -> 123 us); the forward pass keeps them in global memory (with the copy it
+  // needed 171 VGPRs and took 257 us instead of 148)
+  constexpr int TW = ct_twsize(G.outer, INV);
+  cx<float>* twl = reinterpret_cast<cx<float>*>(smem_raw) + S * (INV ? 2 : 1) * G.n1 * G.colT;
+  if constexpr (INV) {
+    const cx<float>* tw_o = f.twp + f.pa.twoff[1][0];
+    for (int i = threadIdx.x; i < TW; i += blockDim.x) twl[i] = tw_o[i];
+    __syncthreads();
+  }
   if constexpr (S > 1) {
     const int sub = threadIdx.x / G.outNT;
-    body_outer2d<G, INV, INV, true>(f, blockIdx.x * S + sub, threadIdx.x - sub * G.outNT,
-                                    smem_raw + (size_t)sub * (INV ? 2 : 1) * G.n1 * G.colT * 8);
+    body_outer2d<G, INV, INV, true, INV>(
+        f, blockIdx.x * S + sub, threadIdx.x - sub * G.outNT,
+        smem_raw + (size_t)sub * (INV ? 2 : 1) * G.n1 * G.colT * 8, twl);
   } else {
-    body_outer2d<G, INV, INV>(f, blockIdx.x, threadIdx.x, smem_raw);
+    body_outer2d<G, INV, INV, false, INV>(f, blockIdx.x, threadIdx.x, smem_raw, twl);
   }
 }
 
@@ -84,15 +95,24 @@ static hipError_t ct_launch(const Flow<float>& f, int which, int arg, hipStream_
       auto fn = kc_line2d<N>;
       static const bool attr_ = set_lds_max((const void*)fn);
       (void)attr_;
-      size_t lds = (size_t)G.lineT * ct_rows_pitch(N) * 8;
+      // row tiles and the near passes' twiddle tables
+      size_t lds = ((size_t)G.lineT * RowsCT<N, G.lineT>::LP + twl_line2d_elems<G>()) * 8;
       const size_t red = (size_t)2 * G.lineNT * sizeof(double);
       if (lds < red) lds = red;
       hipLaunchKernelGGL(fn, dim3(N / G.lineT), dim3(G.lineNT), lds, st, f, arg);
       break;
     }
     case 1: {
-      const size_t lds = (size_t)(kMs2<N> ? 2 : 1) * G.n2 * G.colT * 8;
+      // transform regions and the pass twiddles of both directions
+      const size_t lds = ((size_t)(kMs2<N> ? 2 : 1) * G.n2 * G.colT + ct_twsize(G.spec, false) +
+                          ct_twsize(G.spec, true)) * 8;
       const dim3 grid(G.n1 * NJT), block(G.specNT);
+      static const bool attr_ = set_lds_max((const void*)kc_spec2d<N, kStagePre>) &&
+                                set_lds_max((const void*)kc_spec2d<N, 0>) &&
+                                set_lds_max((const void*)kc_spec2d<N, 1>) &&
+                                set_lds_max((const void*)kc_spec2d<N, 2>) &&
+                                set_lds_max((const void*)kc_spec2d<N, 3>);
+      (void)attr_;
       switch (arg) {
         case kStagePre: hipLaunchKernelGGL((kc_spec2d<N, kStagePre>), grid, block, lds, st, f); break;
         case 0: hipLaunchKernelGGL((kc_spec2d<N, 0>), grid, block, lds, st, f); break;
@@ -113,7 +133,8 @@ static hipError_t ct_launch(const Flow<float>& f, int which, int arg, hipStream_
           hipLaunchKernelGGL(fn, dim3(G.n2 * NJT / S), dim3(G.outNT * S), lds, st, f);
         } else {
           constexpr int S = kOuterS<N, true>;
-          const size_t lds = (size_t)S * 2 * G.n1 * G.colT * 8;  // two sequence regions
+          // two sequence regions per tile and the pass twiddles
+          const size_t lds = ((size_t)S * 2 * G.n1 * G.colT + ct_twsize(G.outer, true)) * 8;
           auto fn = kc_outer2d<N, true>;
           static const bool attr_ = set_lds_max((const void*)fn);
           (void)attr_;
