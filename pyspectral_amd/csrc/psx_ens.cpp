@@ -309,10 +309,14 @@ int psx_ens_create(int count, const psx_flow_params* params, const int* variants
     for (int t = 0; t < g.n1 * njt; ++t) items[1].push_back({m, t});
     if (g.n1 > 1)
       for (int t = 0; t < g.n2 * njt / g.outS; ++t) items[2].push_back({m, t});
-    const size_t line_lds = std::max((size_t)g.lineT * ct_rows_pitch(n) * 8,
+    // row or column tiles plus the pass-twiddle copies the bodies keep in LDS
+    const size_t line_tw = (size_t)ct_twoff(g.line, false, g.line.np - 1) +
+                           ct_twoff(g.line, true, g.line.np - 1);
+    const size_t line_lds = std::max(((size_t)g.lineT * ct_rows_pitch(n) + line_tw) * 8,
                                      (size_t)2 * kEnsNT * sizeof(double));
     e->lds[0] = std::max(e->lds[0], line_lds);
-    e->lds[1] = std::max(e->lds[1], (size_t)g.n2 * g.colT * 8);
+    e->lds[1] = std::max(e->lds[1], ((size_t)g.n2 * g.colT + ct_twsize(g.spec, false) +
+                                     ct_twsize(g.spec, true)) * 8);
     if (g.n1 > 1) e->lds[2] = std::max(e->lds[2], (size_t)g.outS * g.n1 * g.colT * 8);
     if (launch_params(e->prm + e->prm_off[m], nh, params[m].nu, 0.0, params[m].alpha, 0.0, params[m].klo2,
                       params[m].khi2, params[m].ka2, nullptr) != hipSuccess)

@@ -1,5 +1,6 @@
 // Persistent multi-size ensemble kernels (psx_ens.h): one launch per phase for every flow
-// of an ensemble.  Work-group b handles items b, b + gridDim.x, ...; each item runs the body
+// of an ensemble.  The line and spectral bodies read their pass twiddles from LDS copies made
+// per item (forced ensemble: spectral phase 285 -> 257 us).  Work-group b handles items b, b + gridDim.x, ...; each item runs the body
 // of psx_k2d_body.h compiled for its flow's (size, spectral variant), so flows of different
 // sizes share one launch and the launch never waits on a smaller grid's tail.
 #include "psx_ens.h"
@@ -21,9 +22,9 @@ __device__ __forceinline__ void ens_item(const EnsFlow& ef, int tile, int tid, i
   case ens_code(NV, VV): {                                                             \
     constexpr Geo2D G = GE<NV, VV>::g;                                                 \
     if constexpr (KIND == 0) {                                                         \
-      body_line2d<G>(ef.f, tile, tid, flags, smem);                                    \
+      body_line2d<G, true>(ef.f, tile, tid, flags, smem);                                    \
     } else if constexpr (KIND == 1) {                                                  \
-      body_spec2d<G, ARG>(ef.f, tile, tid, smem);                                      \
+      body_spec2d<G, ARG, false, true>(ef.f, tile, tid, smem);                                      \
     } else if constexpr (G.n1 > 1) {                                                   \
       const int sub = tid / G.outNT;                                                   \
       body_outer2d<G, KIND == 3>(ef.f, tile * G.outS + sub, tid % G.outNT,             \
