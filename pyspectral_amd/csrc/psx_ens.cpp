@@ -316,7 +316,8 @@ int psx_ens_create(int count, const psx_flow_params* params, const int* variants
                                      (size_t)2 * kEnsNT * sizeof(double));
     e->lds[0] = std::max(e->lds[0], line_lds);
     e->lds[1] = std::max(e->lds[1], ((size_t)g.n2 * g.colT + ct_twsize(g.spec, false) +
-                                     ct_twsize(g.spec, true)) * 8);
+                                     ct_twsize(g.spec, true) + (g.n1 > 1 ? g.n2 : 0)) * 8 +
+                                        (size_t)2 * (g.n2 + g.colT) * sizeof(double));
     if (g.n1 > 1) e->lds[2] = std::max(e->lds[2], (size_t)g.outS * g.n1 * g.colT * 8);
     if (launch_params(e->prm + e->prm_off[m], nh, params[m].nu, 0.0, params[m].alpha, 0.0, params[m].klo2,
                       params[m].khi2, params[m].ka2, nullptr) != hipSuccess)

@@ -161,10 +161,33 @@ __device__ __forceinline__ void body_spec2d(const Flow<float>& f, int tile, int 
   cx<T>* W1 = f.W + FS;
   const cx<T>* tw_fwd = f.twp + f.pcol.twoff[0][0];
   const cx<T>* tw_rev = f.twp + f.pcol.twoff[1][0];
+  // TWL: LDS copies of the pass twiddles, of the FP64 integrating-factor table entries of this
+  // tile's rows (|kx|) and columns (ky) (ifl[0, L) = t1(|kx|), [L, L + CT) = t1(ky), then the
+  // same for the half-step table t_1/2) and, four-step, of the tile's w^(e grp) twiddles.
+  // Read from global memory after the transforms, their latency sat on the critical path
+  // (8192^2 spectral kernel 246 -> 209 us with the pass twiddles and factors).
+  [[maybe_unused]] double* ifl = nullptr;
   if constexpr (TWL) {
     constexpr int TWF = ct_twsize(G.spec, false), TWR = ct_twsize(G.spec, true);
     cx<T>* twl = lds + (MS ? 2 : 1) * L * CT;
     for (int i = tid; i < TWF + TWR; i += NT) twl[i] = i < TWF ? tw_fwd[i] : tw_rev[i - TWF];
+    ifl = reinterpret_cast<double*>(twl + TWF + TWR);
+    for (int i = tid; i < 2 * (L + CT); i += NT) {
+      const int h = i < L + CT ? 0 : 1, r = i - h * (L + CT);
+      int idx;
+      if (r < L) {
+        const int k = FOUR ? grp + N1 * r : r;
+        const int kx = k < N / 2 ? k : k - N;
+        idx = kx < 0 ? -kx : kx;
+      } else {
+        idx = j0 + (r - L);
+      }
+      ifl[i] = f.prm[h * NH + (idx < NH ? idx : NH - 1)];
+    }
+    if constexpr (FOUR) {  // the four-step twiddles w^(e grp) of the tile's rows
+      cx<T>* twg = reinterpret_cast<cx<T>*>(ifl + 2 * (L + CT));
+      for (int e = tid; e < L; e += NT) twg[e] = f.tw[e * grp];
+    }
     __syncthreads();
     tw_fwd = twl;
     tw_rev = twl + TWF;
@@ -246,7 +269,22 @@ __device__ __forceinline__ void body_spec2d(const Flow<float>& f, int tile, int 
         const T fx = (T)kx, fy = (T)ky;
         const cx<T> Nl = cadd(cscl(fx * fy, va[i]), cscl(fx * fx - fy * fy, vb[i]));
         double E1, Eh;
-        if_factors(f.prm, NH, s, ax, ky, -1, k2, E1, Eh);
+        if constexpr (TWL) {
+          // the operations of if_factors on the LDS copies of the table entries
+          E1 = ifl[e] * ifl[L + t];
+          Eh = ifl[L + CT + e] * ifl[2 * L + CT + t];
+          const double dk2 = (double)k2;
+          if (dk2 <= s.ka2) {
+            E1 *= s.ea1;
+            Eh *= s.eah;
+          }
+          if (s.band && dk2 >= s.klo2 && dk2 <= s.khi2) {
+            E1 *= s.eb1;
+            Eh *= s.ebh;
+          }
+        } else {
+          if_factors(f.prm, NH, s, ax, ky, -1, k2, E1, Eh);
+        }
         cx<T> acc = czero<T>();
         if constexpr (kEarlyAcc) {
           acc = aa[i];
@@ -277,7 +315,9 @@ __device__ __forceinline__ void body_spec2d(const Flow<float>& f, int tile, int 
     ct_elem<RS, L, NT, Lay>(i, tid, e, t);
     const long long a = (long long)(eo + e) * LDW + j0 + t;
     if constexpr (FOUR) {
-      const cx<T> w = f.tw[e * grp];
+      cx<T> w;
+      if constexpr (TWL) w = reinterpret_cast<const cx<T>*>(ifl + 2 * (L + CT))[e];
+      else w = f.tw[e * grp];
       W0[a] = cmulc(va[i], w);
       W1[a] = cmulc(vb[i], w);
     } else {

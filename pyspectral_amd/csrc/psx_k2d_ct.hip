@@ -103,9 +103,11 @@ static hipError_t ct_launch(const Flow<float>& f, int which, int arg, hipStream_
       break;
     }
     case 1: {
-      // transform regions and the pass twiddles of both directions
+      // transform regions, the pass twiddles of both directions, the tile's four-step twiddles
+      // and its FP64 integrating-factor table entries
       const size_t lds = ((size_t)(kMs2<N> ? 2 : 1) * G.n2 * G.colT + ct_twsize(G.spec, false) +
-                          ct_twsize(G.spec, true)) * 8;
+                          ct_twsize(G.spec, true) + (G.n1 > 1 ? G.n2 : 0)) * 8 +
+                         (size_t)2 * (G.n2 + G.colT) * sizeof(double);
       const dim3 grid(G.n1 * NJT), block(G.specNT);
       static const bool attr_ = set_lds_max((const void*)kc_spec2d<N, kStagePre>) &&
                                 set_lds_max((const void*)kc_spec2d<N, 0>) &&
