@@ -114,6 +114,65 @@ __device__ __forceinline__ void fft_ct(cx<T> (&v)[E], cx<T>* lds,
   });
 }
 
+// Radix-R pass whose twiddles were loaded beforehand into w[b][m - 1] (as ct_pass otherwise).
+template <int E, int R, bool INV, int L, int Ns, int NT, class Lay, bool FROM, typename T>
+__device__ __forceinline__ void ct_pass_w(cx<T> (&v)[E], cx<T>* lds,
+                                          const cx<T> (&w)[E / R][R - 1], int tid) {
+  constexpr int NB = E / R;
+  constexpr int Lr = L / R;
+  if constexpr (FROM) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      int j, t;
+      Lay::template decomp<Lr>(tid + b * NT, j, t);
+#pragma unroll
+      for (int m = 0; m < R; ++m) v[b * R + m] = lds[Lay::addr(j + m * Lr, t)];
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+#pragma unroll
+    for (int m = 1; m < R; ++m)
+      v[b * R + m] = INV ? cmulc(v[b * R + m], w[b][m - 1]) : cmul(v[b * R + m], w[b][m - 1]);
+    dft<R, INV>(&v[b * R]);
+  }
+}
+
+// As fft_ct_tw2 with QG = np - 1, the last pass's twiddles (from tw_far) loaded at the start of
+// the pass before it, so that their latency overlaps that pass and its exchange.
+template <int E, bool INV, int NT, class Lay, CtPlan P, typename T>
+__device__ __forceinline__ void fft_ct_tw2_pre(cx<T> (&v)[E], cx<T>* lds,
+                                               const cx<T>* __restrict__ tw_near,
+                                               const cx<T>* __restrict__ tw_far, int tid) {
+  constexpr int QL = P.np - 1;
+  constexpr int RL = ct_radix(P, INV, QL);
+  constexpr int NsL = ct_ns(P, INV, QL);
+  constexpr int NBL = E / RL;
+  static_assert(QL >= 1 && NsL > 1, "a last pass with twiddles");
+  cx<T> wl[NBL][RL - 1];
+  static_for<0, P.np>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    constexpr int R = ct_radix(P, INV, q);
+    constexpr int Ns = ct_ns(P, INV, q);
+    constexpr int off = ct_twoff(P, INV, q);
+    if constexpr (q == QL - 1) {
+      const cx<T>* twl_g = tw_far + ct_twoff(P, INV, QL);
+#pragma unroll
+      for (int b = 0; b < NBL; ++b) {
+        int j, t;
+        Lay::template decomp<P.L / RL>(tid + b * NT, j, t);
+        const int k = j % NsL;
+#pragma unroll
+        for (int m = 1; m < RL; ++m) wl[b][m - 1] = twl_g[(m - 1) * NsL + k];
+      }
+    }
+    if constexpr (q < QL)
+      ct_pass<E, R, INV, P.L, Ns, NT, Lay>(v, lds, q > 0, true, tw_near + off, tid);
+    else
+      ct_pass_w<E, R, INV, P.L, Ns, NT, Lay, (q > 0)>(v, lds, wl, tid);
+  });
+}
+
 // As fft_ct, with the twiddles of passes q < QG read from tw_near (a copy of the start of the
 // order's table, e.g. in LDS) and those of the later passes from tw_far.
 template <int E, bool INV, int NT, class Lay, CtPlan P, int QG, typename T>

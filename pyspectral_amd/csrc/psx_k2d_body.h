@@ -95,7 +95,12 @@ __device__ __forceinline__ void body_line2d(const Flow<float>& f, int tile, int 
     });
   }
   if constexpr (TWL) {
-    fft_ct_tw2<E, true, NT, Lay, G.line, QG>(v, lds, twl + TWNF, tw_rev, tid);
+    // 8192-point rows load the last pass's twiddles one pass early (line kernel 263 -> 259.5
+    // us); at 4096 the extra registers would drop it to one wave per SIMD
+    if constexpr (L == 8192)
+      fft_ct_tw2_pre<E, true, NT, Lay, G.line>(v, lds, twl + TWNF, tw_rev, tid);
+    else
+      fft_ct_tw2<E, true, NT, Lay, G.line, QG>(v, lds, twl + TWNF, tw_rev, tid);
   } else {
     fft_ct<E, true, NT, Lay, G.line>(v, lds, tw_rev, tid);
   }
@@ -118,7 +123,10 @@ __device__ __forceinline__ void body_line2d(const Flow<float>& f, int tile, int 
     v[i] = mkc((w - u) * (w + u), u * w);
   }
   if constexpr (TWL) {
-    fft_ct_tw2<E, false, NT, Lay, G.line, QG>(v, lds, twl, tw_fwd, tid);
+    if constexpr (L == 8192)
+      fft_ct_tw2_pre<E, false, NT, Lay, G.line>(v, lds, twl, tw_fwd, tid);
+    else
+      fft_ct_tw2<E, false, NT, Lay, G.line, QG>(v, lds, twl, tw_fwd, tid);
   } else {
     fft_ct<E, false, NT, Lay, G.line>(v, lds, tw_fwd, tid);
   }
