@@ -23,6 +23,22 @@ struct RowsCT {
   static constexpr int L = L_, T = T_;
   static constexpr int LP = kPad32 ? L_ + (L_ >> 5) + 1 : ct_rows_pitch(L_);  // <= the latter
   static constexpr PSX_HD int addr(int e, int t) { return t * LP + e + (e >> (kPad32 ? 5 : 4)); }
+  // Address in the exchange after pass X of the forward (INV = false) or reversed plan, as the
+  // multi-sequence transforms use it.  Four 384-point rows per tile (plan 4 4 2 12, 12
+  // elements per thread, 32-lane half-waves within one row): three of the six exchanges take
+  // an XOR swizzle of the element index within aligned 32- or 64-element blocks in place of
+  // the pad, so that the Stockham writes of the pass (a thread's outputs Ns elements apart)
+  // meet no bank conflicts while the next pass's reads (32 consecutive elements per
+  // half-wave) stay conflict-free.  The other three keep the pad: the swizzles that halve
+  // their remaining two-way write conflicts cost more address arithmetic than they save.
+  static constexpr bool kSwz384 = L_ == 384 && T_ == 4;
+  template <bool INV, int X>
+  static constexpr PSX_HD int xaddr(int e, int t) {
+    if constexpr (kSwz384 && INV && X == 0) return t * LP + (e ^ ((e >> 5) & 3));
+    else if constexpr (kSwz384 && !INV && X == 1) return t * LP + (e ^ ((e >> 3) & 15));
+    else if constexpr (kSwz384 && !INV && X == 2) return t * LP + (e ^ ((e >> 1) & 16));
+    else return addr(e, t);
+  }
   template <int Lr>
   static constexpr PSX_HD void decomp(int g, int& j, int& t) {
     t = g / Lr;
@@ -30,11 +46,30 @@ struct RowsCT {
   }
 };
 
+// True when the addresses of every exchange X < NX of a row layout map the L elements of row 0
+// one-to-one into [0, LP) (a swizzle must stay a permutation within the padded row).
+template <class Lay, bool INV, int NX>
+constexpr bool ct_xaddr_ok() {
+  if constexpr (NX == 0) {
+    return true;
+  } else {
+    bool seen[Lay::LP] = {};
+    for (int e = 0; e < Lay::L; ++e) {
+      const int a = Lay::template xaddr<INV, NX - 1>(e, 0);
+      if (a < 0 || a >= Lay::LP || seen[a]) return false;
+      seen[a] = true;
+    }
+    return ct_xaddr_ok<Lay, INV, NX - 1>();
+  }
+}
+
 // Column tiles: T_ columns (power of two) side by side; lanes walk the columns.
 template <int T_>
 struct ColsCT {
   static constexpr int T = T_;
   static constexpr PSX_HD int addr(int e, int t) { return e * T + t; }
+  template <bool INV, int X>
+  static constexpr PSX_HD int xaddr(int e, int t) { return addr(e, t); }
   template <int Lr>
   static constexpr PSX_HD void decomp(int g, int& j, int& t) {
     t = g % T;
@@ -196,9 +231,10 @@ __device__ __forceinline__ void fft_ct_tw2(cx<T> (&v)[E], cx<T>* lds,
 // twiddle once for all NS sequences, and the NS exchanges of a pass share one pair of
 // barriers.  Run one after another, each transform pays its own twiddle-load and barrier
 // latency; together, the loads and LDS accesses of the NS sequences are in flight at once.
-// v[0 .. NS-1] are transformed (NS <= NA).
+// v[0 .. NS-1] are transformed (NS <= NA).  Q is the pass's index in the plan: the pass reads
+// the exchange after pass Q - 1 and writes the one after pass Q (the layout's xaddr).
 template <int NS, int NA, int E, int R, bool INV, int L, int Ns, int NT, class Lay, int STRIDE,
-          bool FROM, bool TO, bool WS = false, typename T>
+          bool FROM, bool TO, bool WS = false, int Q = 0, typename T>
 __device__ __forceinline__ void ct_pass_multi(cx<T> (&v)[NA][E], cx<T>* lds,
                                               const cx<T>* __restrict__ tw, int tid) {
   static_assert(E % R == 0, "radix must divide the per-thread element count");
@@ -213,7 +249,8 @@ __device__ __forceinline__ void ct_pass_multi(cx<T> (&v)[NA][E], cx<T>* lds,
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
 #pragma unroll
-        for (int m = 0; m < R; ++m) v[s][b * R + m] = lds[s * STRIDE + Lay::addr(j + m * Lr, t)];
+        for (int m = 0; m < R; ++m)
+          v[s][b * R + m] = lds[s * STRIDE + Lay::template xaddr<INV, Q - 1>(j + m * Lr, t)];
       }
     }
   }
@@ -245,7 +282,8 @@ __device__ __forceinline__ void ct_pass_multi(cx<T> (&v)[NA][E], cx<T>* lds,
 #pragma unroll
       for (int s = 0; s < NS; ++s) {
 #pragma unroll
-        for (int q = 0; q < R; ++q) lds[s * STRIDE + Lay::addr(base + q * Ns, t)] = v[s][b * R + q];
+        for (int q = 0; q < R; ++q)
+          lds[s * STRIDE + Lay::template xaddr<INV, Q>(base + q * Ns, t)] = v[s][b * R + q];
       }
     }
     ct_sync<WS>();
@@ -261,7 +299,7 @@ __device__ __forceinline__ void fft_ct_multi(cx<T> (&v)[NA][E], cx<T>* lds,
     constexpr int R = ct_radix(P, INV, q);
     constexpr int Ns = ct_ns(P, INV, q);
     constexpr int off = ct_twoff(P, INV, q);
-    ct_pass_multi<NS, NA, E, R, INV, P.L, Ns, NT, Lay, STRIDE, (q > 0), (q < P.np - 1), WS>(
+    ct_pass_multi<NS, NA, E, R, INV, P.L, Ns, NT, Lay, STRIDE, (q > 0), (q < P.np - 1), WS, q>(
         v, lds, tw_order + off, tid);
   });
 }

@@ -13,12 +13,13 @@ struct G2 {
   static_assert(g.ok, "no 2D plan for this size");
 };
 
-// The spectral kernel of the four-step sizes and the inverse outer pass transform both fields
-// of a direction together (two LDS regions; body_spec2d, body_outer2d): 8192^2 spectral
-// 278 -> 245 us, inverse outer 158 -> 148 us.  The forward outer pass keeps one region: its
-// 64-thread work-groups at 16 KB each are held below four waves per SIMD by the LDS
-// (154 -> 161 us with two).  One-pass sizes (whole columns of up to 64 KB per work-group) keep
-// one region: two would halve their work-groups per CU.
+// The spectral kernel of the four-step sizes transforms both fields of a direction together
+// (two LDS regions; body_spec2d): 8192^2 spectral 278 -> 245 us.  The outer passes keep one
+// region: their forward work-groups are held below four waves per SIMD by the LDS with two
+// (154 -> 161 us), and with several one-wave tiles per work-group the inverse gained from two
+// regions only before (158 -> 148 us with one tile per work-group, 124.5 against 121.0 us
+// with eight).  One-pass sizes (whole columns of up to 64 KB per work-group) keep one region:
+// two would halve their work-groups per CU.
 template <int N>
 constexpr bool kMs2 = G2<N>::g.n1 > 1;
 
@@ -56,9 +57,11 @@ __global__ void __launch_bounds__((G2<N>::g.outNT > 0 ? G2<N>::g.outNT : 64) * (
   constexpr int S = kOuterS<N, INV>;
   // the inverse pass reads its pass twiddles from one LDS copy for the work-group's tiles
   // (8192^2: 128 -> 123 us); the forward pass keeps them in global memory (with the copy it
-  // needed 171 VGPRs and took 257 us instead of 148)
+  // needed 171 VGPRs and took 257 us instead of 148).  Both directions run a tile's two
+  // transforms one after the other through one LDS region (8192^2 inverse: 124.5 -> 121.0 us
+  // against two regions; 125 us with 6 tiles per work-group, 137 with 12)
   constexpr int TW = ct_twsize(G.outer, INV);
-  cx<float>* twl = reinterpret_cast<cx<float>*>(smem_raw) + S * (INV ? 2 : 1) * G.n1 * G.colT;
+  cx<float>* twl = reinterpret_cast<cx<float>*>(smem_raw) + S * G.n1 * G.colT;
   if constexpr (INV) {
     const cx<float>* tw_o = f.twp + f.pa.twoff[1][0];
     for (int i = threadIdx.x; i < TW; i += blockDim.x) twl[i] = tw_o[i];
@@ -66,9 +69,9 @@ __global__ void __launch_bounds__((G2<N>::g.outNT > 0 ? G2<N>::g.outNT : 64) * (
   }
   if constexpr (S > 1) {
     const int sub = threadIdx.x / G.outNT;
-    body_outer2d<G, INV, INV, true, INV>(
+    body_outer2d<G, INV, false, true, INV>(
         f, blockIdx.x * S + sub, threadIdx.x - sub * G.outNT,
-        smem_raw + (size_t)sub * (INV ? 2 : 1) * G.n1 * G.colT * 8, twl);
+        smem_raw + (size_t)sub * G.n1 * G.colT * 8, twl);
   } else {
     body_outer2d<G, INV, INV, false, INV>(f, blockIdx.x, threadIdx.x, smem_raw, twl);
   }
@@ -135,8 +138,8 @@ static hipError_t ct_launch(const Flow<float>& f, int which, int arg, hipStream_
           hipLaunchKernelGGL(fn, dim3(G.n2 * NJT / S), dim3(G.outNT * S), lds, st, f);
         } else {
           constexpr int S = kOuterS<N, true>;
-          // two sequence regions per tile and the pass twiddles
-          const size_t lds = ((size_t)S * 2 * G.n1 * G.colT + ct_twsize(G.outer, true)) * 8;
+          // one transform region per tile and the pass twiddles
+          const size_t lds = ((size_t)S * G.n1 * G.colT + ct_twsize(G.outer, true)) * 8;
           auto fn = kc_outer2d<N, true>;
           static const bool attr_ = set_lds_max((const void*)fn);
           (void)attr_;

@@ -73,6 +73,8 @@ __global__ void __launch_bounds__(G3<N>::g.lineNT, 1) kc_line3d(Flow<float> f, i
   constexpr int NF = ROT ? 6 : 5;
   constexpr int TPW = line3d_tpw<N, ROT>();
   using Lay = RowsCT<L, RT>;
+  static_assert(ct_xaddr_ok<Lay, false, G.line.np - 1>() && ct_xaddr_ok<Lay, true, G.line.np - 1>(),
+                "exchange layouts");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<T>* lds = reinterpret_cast<cx<T>*>(smem_raw);
   const int tid = threadIdx.x;
