@@ -614,7 +614,7 @@ def ensemble_bench(a: argparse.Namespace, cpu: bool) -> dict[str, Any]:
     states = [initial.random_phase_2d(c.n, k0=6.0, seed=1000 + m, dtype=np.complex64)
               for m, c in enumerate(cfgs)]
     t0 = time.perf_counter()
-    tuned = autotune(cfgs, nsteps=3)
+    tuned = autotune(cfgs, nsteps=5)
     tune_s = time.perf_counter() - t0
     ens = EnsembleSolver(cfgs, tuned["variants"], wg_per_cu=tuned["wg_per_cu"])
     ens.set_states(states)

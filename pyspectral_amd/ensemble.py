@@ -17,6 +17,7 @@ device; every candidate computes the same transforms, so the choice changes run 
 from __future__ import annotations
 
 import ctypes
+import gc
 import hashlib
 import json
 import math
@@ -69,18 +70,33 @@ class EnsembleSolver:
             [variants_for(c.n)[0] if variants_for(c.n) else -1 for c in cfgs]
         self.params = (_lib.FlowParams * self.count)(*[flow_params(c, "basdevant")
                                                        for c in cfgs])
-        self.u0_rows, self.u0, self.acc, self.work = [], [], [], []
-        for m, c in enumerate(cfgs):
+        lays = []
+        for m in range(self.count):
             lay = _lib.Layout()
             if self.lib.psx_ens_query_layout(ctypes.byref(self.params[m]), self.variants[m],
                                              ctypes.byref(lay)) != 0:
                 raise BackendUnavailable(self.lib.psx_last_error().decode())
-            nh = c.n // 2 + 1
-            rows = torch.zeros((c.n, lay.state_pitch), dtype=torch.complex64, device="cuda")
-            self.u0_rows.append(rows)
-            self.u0.append(rows[:, :nh])
-            self.acc.append(torch.zeros_like(rows))
-            self.work.append(torch.zeros(lay.work_elems, dtype=torch.complex64, device="cuda"))
+            lays.append(lay)
+        # one arena per kind of buffer (state rows, accumulator, work), each member's slice at a
+        # 64 KiB-aligned offset: the members' placement in HBM is then the same whatever the
+        # process allocated and freed before (separate tensors from the caching allocator left
+        # the line phase 5-15 % slower after an autotuning pass than in a fresh process)
+        align = 8192  # complex64 elements in 64 KiB
+
+        def arena(sizes: list[int]) -> list[Any]:
+            offs, total = [], 0
+            for sz in sizes:
+                offs.append(total)
+                total += -(-sz // align) * align
+            buf = torch.zeros(total, dtype=torch.complex64, device="cuda")
+            return [buf[o:o + sz] for o, sz in zip(offs, sizes)]
+
+        row_sizes = [c.n * lay.state_pitch for c, lay in zip(cfgs, lays)]
+        self.u0_rows = [r.view(c.n, lay.state_pitch)
+                        for r, c, lay in zip(arena(row_sizes), cfgs, lays)]
+        self.u0 = [r[:, :c.n // 2 + 1] for r, c in zip(self.u0_rows, cfgs)]
+        self.acc = [a.view(c.n, lay.state_pitch) for a, c, lay in zip(arena(row_sizes), cfgs, lays)]
+        self.work = arena([lay.work_elems for lay in lays])
         ptrs = lambda ts: (ctypes.c_void_p * self.count)(*[t.data_ptr() for t in ts])  # noqa: E731
         var = (ctypes.c_int * self.count)(*self.variants)
         ctx = ctypes.c_void_p()
@@ -254,6 +270,18 @@ def _time_steps(ens: EnsembleSolver, dt: float, nsteps: int, warm: int = 2) -> f
     return (time.perf_counter() - t0) / nsteps
 
 
+def _time_candidates(solvers: dict[int, EnsembleSolver], dt: Any, nsteps: int,
+                     rounds: int = 3) -> dict[int, float]:
+    """Seconds per step of each candidate: the best of ``rounds`` interleaved timings, so that
+    a slow moment of the device (clocks, another process) does not pick the candidate timed
+    during it."""
+    best = {k: math.inf for k in solvers}
+    for _ in range(rounds):
+        for k, ens in solvers.items():
+            best[k] = min(best[k], _time_steps(ens, dt, nsteps))
+    return best
+
+
 def _cache_path() -> Path:
     env = os.environ.get("PSX_AUTOTUNE_CACHE")
     return Path(env) if env else Path.home() / ".cache" / "pyspectral_amd" / "autotune.json"
@@ -301,7 +329,8 @@ def _cache_store(key: str, result: dict[str, Any]) -> None:
 def autotune(cfgs: Sequence[Config], *, nsteps: int = 5, grids: Sequence[int] = (1, 2, 4),
              seed: int = 0, cache: bool = True) -> dict[str, Any]:
     """Choose each member size's spectral variant, then the persistent grid, by timing
-    graph-replayed steps of the compiled candidates on this device.
+    graph-replayed steps of the compiled candidates on this device (the best of three
+    interleaved timings of ``nsteps`` steps each).
 
     Variants are timed on a sub-ensemble holding only that size's members (same count), on a
     random state; the grid is timed on the whole ensemble with the chosen variants.  Returns
@@ -327,6 +356,8 @@ def autotune(cfgs: Sequence[Config], *, nsteps: int = 5, grids: Sequence[int] = 
 
 def _autotune(cfgs: Sequence[Config], *, nsteps: int, grids: Sequence[int],
               seed: int) -> dict[str, Any]:
+    import torch
+
     from pyspectral_amd import initial
 
     by_size: dict[int, list[int]] = {}
@@ -339,23 +370,31 @@ def _autotune(cfgs: Sequence[Config], *, nsteps: int, grids: Sequence[int],
         states = [initial.random_phase_2d(n, k0=min(10.0, n / 12), seed=seed + i)
                   for i in range(len(sub))]
         dt = 0.2 * (2 * math.pi / n)
-        res = {}
+        solvers = {}
         for v in variants_for(n):
-            ens = EnsembleSolver(sub, [v] * len(sub))
-            ens.set_states(states)
-            res[v] = _time_steps(ens, dt, nsteps)
+            solvers[v] = EnsembleSolver(sub, [v] * len(sub))
+            solvers[v].set_states(states)
+        res = _time_candidates(solvers, dt, nsteps)
+        for ens in solvers.values():
             ens.close()
         best_var[n] = min(res, key=res.get)
         timings["variants"][n] = {VARIANT_NAMES[v]: t * 1e3 for v, t in res.items()}
     variants = [best_var[c.n] for c in cfgs]
     states = [initial.random_phase_2d(c.n, k0=min(10.0, c.n / 12), seed=seed + m)
               for m, c in enumerate(cfgs)]
-    gres = {}
+    solvers = {}
     for g in grids:
-        ens = EnsembleSolver(cfgs, variants, wg_per_cu=g)
-        ens.set_states(states)
-        gres[g] = _time_steps(ens, [0.2 * (2 * math.pi / c.n) for c in cfgs], nsteps)
+        solvers[g] = EnsembleSolver(cfgs, variants, wg_per_cu=g)
+        solvers[g].set_states(states)
+    gres = _time_candidates(solvers, [0.2 * (2 * math.pi / c.n) for c in cfgs], nsteps)
+    for ens in solvers.values():
         ens.close()
+    # hand the candidates' buffers back to the device, so that the tuned ensemble's buffers
+    # are allocated afresh rather than carved out of the cached blocks of the candidates
+    ens = None
+    solvers.clear()
+    gc.collect()
+    torch.cuda.empty_cache()
     best_grid = min(gres, key=gres.get)
     timings["grid"] = {g: t * 1e3 for g, t in gres.items()}
     return {"variants": variants, "wg_per_cu": best_grid, "timings": timings}
