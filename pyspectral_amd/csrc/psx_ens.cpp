@@ -139,6 +139,22 @@ int grid_of(const psx_ens* e, int phase) {
   return std::max(1, std::min(e->nitems[phase], e->wg_per_cu * e->ncu));
 }
 
+// Item order of a column phase: in each block of 64 items, the eight runs of eight consecutive
+// items (neighbouring column tiles of one member) are interleaved, so that the items of a run
+// take positions equal mod 8.  Work-group b of a persistent launch starts on XCD b mod 8 and
+// its items are b, b + grid, ... with the grid a multiple of 8 (or the whole list), so a run's
+// tiles are processed at about the same time on one XCD and the 128-byte lines that
+// neighbouring tiles share (tiles of 4 or 8 columns, rows not aligned to 128 bytes) are
+// fetched into that XCD's L2 once instead of once per tile.
+void xcd_runs(std::vector<EnsItem>& v) {
+  const size_t full = v.size() / 64 * 64;
+  std::vector<EnsItem> out(v);
+  for (size_t b = 0; b < full; b += 64)
+    for (int r = 0; r < 8; ++r)
+      for (int s = 0; s < 8; ++s) out[b + 8 * s + r] = v[b + 8 * r + s];
+  v.swap(out);
+}
+
 int ens_phase(psx_ens* e, int kind, int arg, hipStream_t st) {
   const int ph = kind == 0 ? 0 : (kind == 1 ? 1 : 2);
   if (e->nitems[ph] == 0) return 0;
@@ -328,6 +344,8 @@ int psx_ens_create(int count, const psx_flow_params* params, const int* variants
   if (hipMemcpy(e->dflows, e->hflows.data(), count * sizeof(EnsFlow), hipMemcpyHostToDevice) !=
       hipSuccess)
     return bail(fail("psx_ens_create: copy failed"));
+  xcd_runs(items[1]);
+  xcd_runs(items[2]);
   for (int ph = 0; ph < 3; ++ph) {
     e->nitems[ph] = (int)items[ph].size();
     if (items[ph].empty()) continue;

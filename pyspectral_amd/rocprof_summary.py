@@ -93,9 +93,20 @@ def _calibration(vals: dict[str, list[float]]) -> float:
     return sorted(best)[len(best) // 2]
 
 
-def traffic(fetch: Path, write: Path, kernels: list[str] | None) -> dict:
+def _active(vals: dict[str, list[float]], frac: float) -> dict[str, list[float]]:
+    """Per kernel, the dispatches whose value reaches ``frac`` of the kernel's largest (with
+    frac > 0: drops the smaller launches of the same kernel, such as an autotuner's timings of
+    sub-ensembles, from the mean)."""
+    if frac <= 0:
+        return vals
+    return {k: [v for v in vs if v >= frac * max(vs)] for k, vs in vals.items()}
+
+
+def traffic(fetch: Path, write: Path, kernels: list[str] | None,
+            active_frac: float = 0.0) -> dict:
     fv, wv = counter_means(fetch, "FETCH_SIZE"), counter_means(write, "WRITE_SIZE")
     cal_f, cal_w = _calibration(fv), _calibration(wv)
+    fv, wv = _active(fv, active_frac), _active(wv, active_frac)
     ff = 2.0 ** 30 / (cal_f * 1024.0)
     fw = 2.0 ** 30 / (cal_w * 1024.0)
     res = {"fetch_factor": ff, "write_factor": fw, "kernels": {}}
@@ -140,6 +151,8 @@ def main(argv: list[str] | None = None) -> int:
     p.add_argument("fetch", type=Path)
     p.add_argument("write", type=Path)
     p.add_argument("--kernels", default="")
+    p.add_argument("--active-frac", type=float, default=0.0,
+                   help="keep a kernel's dispatches reaching this fraction of its largest")
     q = sub.add_parser("counters")
     q.add_argument("csv", type=Path)
     q.add_argument("--ref", required=True, help="counter that marks a dispatch active")
@@ -152,7 +165,7 @@ def main(argv: list[str] | None = None) -> int:
     if a.cmd == "trace":
         print(trace_table(a.csv, a.active_us))
         return 0
-    res = traffic(a.fetch, a.write, [s for s in a.kernels.split(",") if s])
+    res = traffic(a.fetch, a.write, [s for s in a.kernels.split(",") if s], a.active_frac)
     print(f"calibration: reads = FETCH_SIZE x {res['fetch_factor']:.4f}, writes = WRITE_SIZE x "
           f"{res['write_factor']:.4f}\n")
     print("| kernel | dispatches | HBM read (MB) | HBM write (MB) | total (MB) |")
