@@ -22,7 +22,22 @@ struct RowsCT {
   static constexpr bool kPad32 = (L_ == 8192 && T_ == 1) || L_ == 384;
   static constexpr int L = L_, T = T_;
   static constexpr int LP = kPad32 ? L_ + (L_ >> 5) + 1 : ct_rows_pitch(L_);  // <= the latter
-  static constexpr PSX_HD int addr(int e, int t) { return t * LP + e + (e >> (kPad32 ? 5 : 4)); }
+  static constexpr int kShift = kPad32 ? 5 : 4;
+  static constexpr PSX_HD int addr(int e, int t) { return t * LP + e + (e >> kShift); }
+  // True when, for every butterfly of a radix-R pass with stride Ns, the addresses of its R
+  // Stockham outputs (elements base + q Ns, base = (j - k) R + k, k = j % Ns) lie at
+  // addr(base) + wdelta(q Ns): the writes then take one address computation per butterfly and
+  // constant offsets instead of a shift and add per element.
+  template <int R, int Ns>
+  static constexpr bool write_folds() {
+    for (int j = 0; j < L / R; ++j) {
+      const int k = j % Ns, base = (j - k) * R + k;
+      for (int q = 1; q < R; ++q)
+        if (addr(base + q * Ns, 0) - addr(base, 0) != wdelta(q * Ns)) return false;
+    }
+    return true;
+  }
+  static constexpr PSX_HD int wdelta(int d) { return d + (d >> kShift); }
   // Address in the exchange after pass X of the forward (INV = false) or reversed plan, as the
   // multi-sequence transforms use it.  Four 384-point rows per tile (plan 4 4 2 12, 12
   // elements per thread, 32-lane half-waves within one row): three of the six exchanges take
@@ -70,6 +85,9 @@ struct ColsCT {
   static constexpr PSX_HD int addr(int e, int t) { return e * T + t; }
   template <bool INV, int X>
   static constexpr PSX_HD int xaddr(int e, int t) { return addr(e, t); }
+  template <int R, int Ns>
+  static constexpr bool write_folds() { return true; }
+  static constexpr PSX_HD int wdelta(int d) { return d * T; }
   template <int Lr>
   static constexpr PSX_HD void decomp(int g, int& j, int& t) {
     t = g % T;
@@ -128,8 +146,14 @@ __device__ __forceinline__ void ct_pass(cx<T> (&v)[E], cx<T>* lds, bool from_lds
       Lay::template decomp<Lr>(tid + b * NT, j, t);
       const int k = j % Ns;
       const int base = (j - k) * R + k;
+      if constexpr (Lay::template write_folds<R, Ns>()) {
+        const int a0 = Lay::addr(base, t);
 #pragma unroll
-      for (int q = 0; q < R; ++q) lds[Lay::addr(base + q * Ns, t)] = v[b * R + q];
+        for (int q = 0; q < R; ++q) lds[a0 + Lay::wdelta(q * Ns)] = v[b * R + q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < R; ++q) lds[Lay::addr(base + q * Ns, t)] = v[b * R + q];
+      }
     }
     ct_sync<WS>();
   }
