@@ -216,13 +216,33 @@ __device__ __forceinline__ void fft_ct_tw2_pre(cx<T> (&v)[E], cx<T>* lds,
     constexpr int off = ct_twoff(P, INV, q);
     if constexpr (q == QL - 1) {
       const cx<T>* twl_g = tw_far + ct_twoff(P, INV, QL);
+      if constexpr (sizeof(T) == 4) {
+        // buffer loads: the table row (m - 1) NsL goes into the instruction's scalar offset,
+        // where a pointer load needs a 64-bit address add per twiddle
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<cx<T>*>(twl_g), 0, (RL - 1) * NsL * 8, 0x00020000);
 #pragma unroll
-      for (int b = 0; b < NBL; ++b) {
-        int j, t;
-        Lay::template decomp<P.L / RL>(tid + b * NT, j, t);
-        const int k = j % NsL;
+        for (int b = 0; b < NBL; ++b) {
+          int j, t;
+          Lay::template decomp<P.L / RL>(tid + b * NT, j, t);
+          const int vo = (j % NsL) * 8;
 #pragma unroll
-        for (int m = 1; m < RL; ++m) wl[b][m - 1] = twl_g[(m - 1) * NsL + k];
+          for (int m = 1; m < RL; ++m) {
+            const int so = (m - 1) * NsL * 8;
+            const unsigned re = __builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, 0);
+            const unsigned im = __builtin_amdgcn_raw_buffer_load_b32(rs, vo + 4, so, 0);
+            wl[b][m - 1] = mkc(__builtin_bit_cast(T, re), __builtin_bit_cast(T, im));
+          }
+        }
+      } else {
+#pragma unroll
+        for (int b = 0; b < NBL; ++b) {
+          int j, t;
+          Lay::template decomp<P.L / RL>(tid + b * NT, j, t);
+          const int k = j % NsL;
+#pragma unroll
+          for (int m = 1; m < RL; ++m) wl[b][m - 1] = twl_g[(m - 1) * NsL + k];
+        }
       }
     }
     if constexpr (q < QL)
