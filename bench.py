@@ -115,9 +115,11 @@ def source_sha(family: str) -> str:
     return h.hexdigest()[:16]
 
 
-def measured_traffic(kernel: str, workload: str, family: str) -> tuple[float | None, str]:
+def measured_traffic(kernel: str, workload: str, family: str,
+                     variants: dict[str, int] | None = None) -> tuple[float | None, str]:
     """HBM bytes per launch from a committed rocprofv3 counter summary
-    (profiles/traffic_<kernel>_<workload>.json), if it was measured on the current sources."""
+    (profiles/traffic_<kernel>_<workload>.json), if it was measured on the current sources
+    (and, where the file names the ensemble variants it was measured with, on these)."""
     f = ROOT / "profiles" / f"traffic_{kernel}_{workload}.json"
     if not f.exists():
         return None, f"no counter measurement ({f.name})"
@@ -126,6 +128,9 @@ def measured_traffic(kernel: str, workload: str, family: str) -> tuple[float | N
     if d.get("source_sha") != sha:
         return None, (f"{f.name} was measured on kernel sources {d.get('source_sha')}, "
                       f"current {sha}: not used")
+    if "variants" in d and d["variants"] != variants:
+        return None, (f"{f.name} was measured with variants {d['variants']}, "
+                      f"this run's are {variants}: not used")
     return float(d["bytes_per_launch"]), f"{f.name} (sources {sha})"
 
 
@@ -660,9 +665,13 @@ def ensemble_bench(a: argparse.Namespace, cpu: bool) -> dict[str, Any]:
             pb["outer_fwd"] += kb["outer_fwd"]
             pb["outer_inv"] += kb["outer_inv"]
     # the line phase's bytes do not depend on the spectral variants or the grid the autotuner
-    # picks, so one counter measurement of the forced ensemble serves every tuning
+    # picks, so one counter measurement of the forced ensemble serves every tuning; the
+    # spectral phase's figure is used only with the variants it was measured with
     rk = roof_kernel(per_kernel, pb)
-    traffic, traffic_src = (measured_traffic(rk, "ens_forced", "ens") if rk == "line"
+    by_size = {str(c.n): int(v) for c, v in zip(cfgs, tuned["variants"])}
+    traffic, traffic_src = (measured_traffic(rk, "ens_forced", "ens",
+                                             by_size if rk == "spec" else None)
+                            if rk in ("line", "spec")
                             else (None, f"not measured for the {rk} phase"))
     roof = roofline_entry(per_kernel, pb, traffic, traffic_src)
     step_bytes = sum(pb[k] * per_kernel[k]["launches"] for k in per_kernel if k in pb) / a.steps
