@@ -2,7 +2,7 @@
 
     python -m pyspectral_amd.rocprof_summary trace RUN_kernel_trace.csv [--active-us 20]
     python -m pyspectral_amd.rocprof_summary traffic FETCH_counter_collection.csv \
-        WRITE_counter_collection.csv [--kernels SUBSTR,...]
+        WRITE_counter_collection.csv [--kernels SUBSTR,...] [--active-frac F]
     python -m pyspectral_amd.rocprof_summary counters RUN_counter_collection.csv --ref COUNTER \
         [--kernels SUBSTR,...]
 
@@ -17,7 +17,9 @@ min and max duration and the total time.
 over its dispatches that moved data, scaled into bytes by a calibration from the same runs: a
 plain device copy of known size (``copyBuffer``; bench.py copies 1 GiB into another 1 GiB 13
 times when it measures the copy bandwidth) reads and writes exactly its size, so reads =
-FETCH_SIZE x (2^30 / its FETCH_SIZE) and writes likewise.
+FETCH_SIZE x (2^30 / its FETCH_SIZE) and writes likewise.  With --active-frac F only the
+dispatches reaching F times the kernel's largest value count (an autotuner's timings of smaller
+launches of the same kernel are left out).
 """
 
 from __future__ import annotations
