@@ -424,23 +424,6 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
   for (int i = tid; i < 2 * NH; i += NT) ifl[i] = f.prm[i];
   __syncthreads();
 
-  // state offsets of this thread's modes (out of range outside the 2/3 mask)
-  unsigned soff[E];
-#pragma unroll
-  for (int i = 0; i < E; ++i) {
-    int e, t;
-    ct_elem<RP, L, NT, Lay>(i, tid, e, t);
-    const int kx = e < N / 2 ? e : e - N;
-    const int ax = kx < 0 ? -kx : kx;
-    const int kz = j0 + t;
-    soff[i] = (3 * ax < N && 3 * ay < N && kz < JM)
-                  ? (unsigned)(((long long)e * N + iy) * SP + kz)
-                  : 0xFFFFFFFFu;
-  }
-  auto so = [&](int i, int c) -> unsigned {
-    return soff[i] == 0xFFFFFFFFu ? kOob : (soff[i] + (unsigned)(c * SST)) * 8u;
-  };
-
   cx<T> a0[E], a1[E], a2reg[kA2Lds ? 1 : E];
   auto A2 = [&](int i) -> cx<T>& {
     if constexpr (kA2Lds) {
@@ -522,15 +505,22 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
       const int kx = e < N / 2 ? e : e - N;
       const int ax = kx < 0 ? -kx : kx;
       const int kz = j0 + t;
-      const cx<T> u0a = ub.load(so(i, 0)), u0b = ub.load(so(i, 1)), u0c = ub.load(so(i, 2));
+      // state offset of this mode (out of range outside the 2/3 mask), formed here rather than
+      // before the transforms, where its E registers were held across them and spilled
+      const bool in_mask = 3 * ax < N && 3 * ay < N && kz < JM;
+      const unsigned soffi = (unsigned)(((long long)e * N + iy) * SP + kz);
+      auto so = [&](int c) -> unsigned {
+        return in_mask ? (soffi + (unsigned)(c * SST)) * 8u : kOob;
+      };
+      const cx<T> u0a = ub.load(so(0)), u0b = ub.load(so(1)), u0c = ub.load(so(2));
       cx<T> ua = u0a, ubv = u0b, uc = u0c;
       if constexpr (stage == kStagePre) {
         const int k2 = kx * kx + ky * ky + kz * kz;
         const T ik2 = k2 > 0 ? __builtin_amdgcn_rcpf((T)k2) : (T)0;
         leray<T>((T)kx, (T)ky, (T)kz, ik2, ua, ubv, uc);
-        ub.store(so(i, 0), ua);
-        ub.store(so(i, 1), ubv);
-        ub.store(so(i, 2), uc);
+        ub.store(so(0), ua);
+        ub.store(so(1), ubv);
+        ub.store(so(2), uc);
       }
       if constexpr (stage != kStagePre) {
         const int k2 = kx * kx + ky * ky + kz * kz;
@@ -538,9 +528,9 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
         const T ik2 = k2 > 0 ? __builtin_amdgcn_rcpf((T)k2) : (T)0;
         if (f.src) {  // uniform: an additive source (state layout, zero outside the mask)
           const BufC32T<kc3_pol(N)> sb(const_cast<cx<T>*>(f.src), (unsigned long long)3 * SST * 8);
-          a0[i] = cadd(a0[i], sb.load(so(i, 0)));
-          a1[i] = cadd(a1[i], sb.load(so(i, 1)));
-          A2(i) = cadd(A2(i), sb.load(so(i, 2)));
+          a0[i] = cadd(a0[i], sb.load(so(0)));
+          a1[i] = cadd(a1[i], sb.load(so(1)));
+          A2(i) = cadd(A2(i), sb.load(so(2)));
         }
         const cx<T> d = cscl(ik2, cadd(cadd(cscl(fx, a0[i]), cscl(fy, a1[i])), cscl(fz, A2(i))));
         const cx<T> n0 = csub(a0[i], cscl(fx, d));
@@ -550,25 +540,25 @@ __global__ void __launch_bounds__(G3<N>::g.xNT, 1) kc_spec3d(Flow<float> f) {
         if_factors(ifl, NH, s, ax, ay, kz, k2, E1, Eh);
         cx<T> c0 = czero<T>(), c1 = czero<T>(), c2 = czero<T>();
         if constexpr (stage >= 1) {
-          c0 = ab.load(so(i, 0));
-          c1 = ab.load(so(i, 1));
-          c2 = ab.load(so(i, 2));
+          c0 = ab.load(so(0));
+          c1 = ab.load(so(1));
+          c2 = ab.load(so(2));
         }
         ua = rk_value<stage, T>(dt, (T)(1.0 - E1), (T)(1.0 - Eh), n0, u0a, c0);
         ubv = rk_value<stage, T>(dt, (T)(1.0 - E1), (T)(1.0 - Eh), n1, u0b, c1);
         uc = rk_value<stage, T>(dt, (T)(1.0 - E1), (T)(1.0 - Eh), n2, u0c, c2);
         if constexpr (stage < 3) {
-          ab.store(so(i, 0), c0);
-          ab.store(so(i, 1), c1);
-          ab.store(so(i, 2), c2);
+          ab.store(so(0), c0);
+          ab.store(so(1), c1);
+          ab.store(so(2), c2);
         } else {
           leray<T>(fx, fy, fz, ik2, ua, ubv, uc);
-          ub.store(so(i, 0), ua);
-          ub.store(so(i, 1), ubv);
-          ub.store(so(i, 2), uc);
+          ub.store(so(0), ua);
+          ub.store(so(1), ubv);
+          ub.store(so(2), uc);
         }
       }
-      if (soff[i] == 0xFFFFFFFFu) {  // 2/3 mask: no nonlinear term outside it
+      if (!in_mask) {  // 2/3 mask: no nonlinear term outside it
         ua = ubv = uc = czero<T>();
       }
       a0[i] = cscl(scale, ua);
