@@ -86,6 +86,9 @@ struct psx_ens {
   EnsItem* ditems[3] = {nullptr, nullptr, nullptr};  // line, spectral, outer
   int nitems[3] = {0, 0, 0};
   size_t lds[3] = {0, 0, 0};
+  // resident work-groups per CU of each phase kernel: line, spectral stages (pre, 0-3),
+  // outer forward, outer inverse (0: unknown)
+  int res[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   double* prm = nullptr;
   std::vector<long long> prm_off;
   unsigned long long* stats = nullptr;
@@ -135,8 +138,18 @@ int ens_mark(psx_ens* e, int kind, hipStream_t st) {
     if (r_) return r_;                 \
   } while (0)
 
-int grid_of(const psx_ens* e, int phase) {
-  return std::max(1, std::min(e->nitems[phase], e->wg_per_cu * e->ncu));
+int res_slot(int kind, int arg) {
+  return kind == 0 ? 0 : (kind == 1 ? 2 + arg : (kind == 2 ? 6 : 7));  // kStagePre = -1
+}
+
+// Persistent grid of a phase: wg_per_cu work-groups per CU, at most as many as can be resident
+// at once (a work-group that cannot start until another ends would run its items at the end,
+// on an otherwise idle GPU), and no more than there are items.
+int grid_of(const psx_ens* e, int kind, int arg) {
+  const int ph = kind == 0 ? 0 : (kind == 1 ? 1 : 2);
+  const int res = e->res[res_slot(kind, arg)];
+  const int per_cu = res > 0 ? std::min(e->wg_per_cu, res) : e->wg_per_cu;
+  return std::max(1, std::min(e->nitems[ph], per_cu * e->ncu));
 }
 
 // Item order of a column phase: in each block of 64 items, the eight runs of eight consecutive
@@ -158,7 +171,7 @@ void xcd_runs(std::vector<EnsItem>& v) {
 int ens_phase(psx_ens* e, int kind, int arg, hipStream_t st) {
   const int ph = kind == 0 ? 0 : (kind == 1 ? 1 : 2);
   if (e->nitems[ph] == 0) return 0;
-  PSX_HIP(launch_ens(kind, arg, e->dflows, e->ditems[ph], e->nitems[ph], grid_of(e, ph),
+  PSX_HIP(launch_ens(kind, arg, e->dflows, e->ditems[ph], e->nitems[ph], grid_of(e, kind, arg),
                      e->lds[ph], st));
   return 0;
 }
@@ -346,6 +359,15 @@ int psx_ens_create(int count, const psx_flow_params* params, const int* variants
     return bail(fail("psx_ens_create: copy failed"));
   xcd_runs(items[1]);
   xcd_runs(items[2]);
+  {
+    e->res[0] = ens_resident(0, 0, e->lds[0]);
+    for (int st = kStagePre; st <= 3; ++st)
+      e->res[res_slot(1, st)] = ens_resident(1, st, e->lds[1]);
+    if (!items[2].empty()) {
+      e->res[6] = ens_resident(2, 0, e->lds[2]);
+      e->res[7] = ens_resident(3, 0, e->lds[2]);
+    }
+  }
   for (int ph = 0; ph < 3; ++ph) {
     e->nitems[ph] = (int)items[ph].size();
     if (items[ph].empty()) continue;

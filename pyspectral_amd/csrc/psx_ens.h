@@ -28,6 +28,9 @@ constexpr int ens_code(int n, int var) { return n * 8 + var; }
 // forward, 3 outer inverse.
 hipError_t launch_ens(int kind, int arg, const EnsFlow* flows, const EnsItem* items, int nitems,
                       int grid, size_t lds, hipStream_t st);
+// Work-groups of phase (kind, arg) that can be resident on one CU with lds bytes of LDS each
+// (0 if the runtime cannot say).
+int ens_resident(int kind, int arg, size_t lds);
 // Hermitian projection of every member's ky = 0 column after a step (nmax: largest n).
 hipError_t launch_ens_hermitian(const EnsFlow* flows, int count, int nmax, hipStream_t st);
 // True if (n, var) has compiled ensemble bodies.

@@ -104,6 +104,34 @@ static hipError_t ens_go(const EnsFlow* flows, const EnsItem* items, int nitems,
   return hipGetLastError();
 }
 
+template <int KIND, int ARG>
+static int ens_res(size_t lds) {
+  auto fn = ke_phase<KIND, ARG>;
+  static const bool attr_ = set_lds_max((const void*)fn);
+  (void)attr_;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, kEnsNT, lds) != hipSuccess) return 0;
+  return nb;
+}
+
+int ens_resident(int kind, int arg, size_t lds) {
+  switch (kind) {
+    case 0: return ens_res<0, 0>(lds);
+    case 1:
+      switch (arg) {
+        case kStagePre: return ens_res<1, kStagePre>(lds);
+        case 0: return ens_res<1, 0>(lds);
+        case 1: return ens_res<1, 1>(lds);
+        case 2: return ens_res<1, 2>(lds);
+        case 3: return ens_res<1, 3>(lds);
+        default: return 0;
+      }
+    case 2: return ens_res<2, 0>(lds);
+    case 3: return ens_res<3, 0>(lds);
+    default: return 0;
+  }
+}
+
 hipError_t launch_ens(int kind, int arg, const EnsFlow* flows, const EnsItem* items, int nitems,
                       int grid, size_t lds, hipStream_t st) {
   if (nitems <= 0 || grid <= 0) return hipErrorInvalidValue;
