@@ -39,8 +39,11 @@ __device__ __forceinline__ void ens_item(const EnsFlow& ef, int tile, int tid, i
   }
 }
 
+// The spectral phase is held to 168 VGPRs, three work-groups per CU (forced ensemble: 202 ->
+// 178 us; 170-182 VGPRs and two work-groups without the bound); the inverse outer pass would
+// spill at that bound, and the line phase at 129 VGPRs already holds three.
 template <int KIND, int ARG>
-__global__ void __launch_bounds__(kEnsNT, 2)
+__global__ void __launch_bounds__(kEnsNT, KIND == 1 ? 3 : 2)
     ke_phase(const EnsFlow* __restrict__ flows, const EnsItem* __restrict__ items, int nitems,
              int flags) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
